@@ -1,0 +1,147 @@
+// engine.h -- device-resident scenes, buffers and the substep pipeline.
+//
+// An Engine owns one HIP stream and a table of scenes on one device.  The
+// C-ABI's partition is an Engine with one scene (GPU_Partition,
+// gpu_partition.h:79-165); the batch is an Engine with many.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/icempm.h"
+#include "kernels.h"
+#include "mpm_common.h"
+
+namespace icempm {
+
+enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kNumCls };
+
+struct SceneHost {
+    icempm_simparams prms{};
+    bool have_prms = false;
+    int n_pts = 0;            // points currently on the device (incl. disabled until a sort)
+    int cap_pts = 0;
+    long long pt_off = 0;
+    long long cells_off = 0;
+    long long grid_elems = 0; // gpitch
+    void *grid[2] = {nullptr, nullptr};
+    double *ind_force = nullptr;
+    bool clean[2] = {true, true};
+    double sim_time = 0;      // SimulationTime
+    int sim_step = 0;         // SimulationStep
+};
+
+class Engine {
+public:
+    Engine(int device, int precision);
+    ~Engine();
+
+    int device() const { return device_; }
+    int precision() const { return precision_; }
+    int elem_bytes() const { return precision_ == ICEMPM_PRECISION_MIXED ? 4 : 8; }
+    hipStream_t stream() const { return stream_; }
+
+    // set-up
+    int add_scene(const icempm_simparams *prms, int cap_pts);       // returns scene id
+    int set_params(int scene, const icempm_simparams &prms);
+    int set_grid_span(int scene, int xoff, int gx);
+    int commit();                                                    // (re)allocate device memory
+    int ensure_committed();
+    int upload(int scene, const double *hssoa, unsigned capacity, int offset, int n);
+    int download(int scene, double *hssoa, unsigned capacity, int offset, int *n_out);
+
+    // fine-grained phases of one scene on grid buffer 0 (the reference's call sequence)
+    int reset_grid(int scene);
+    int p2g(int scene);
+    int update_nodes(int scene, double t, double ind_x, double ind_y);
+    int g2p(int scene, int recordPQ);
+    int reset_indenter_force();
+    int synchronize();
+
+    // frame-level stepping of all scenes
+    int step(int nsub, int flags);
+    int sort();
+    int autotune(int n_trials, float *out_ms, int n_out, int *chosen);
+
+    // queries
+    int get_indenter_force(int scene, double *out, int n);
+    int get_error_code(uint8_t *code);
+    int get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts, int n_cells);
+    int get_kernel_stats(double *ms, long long *count, int n);
+    int get_grid(int scene, int buf, double *out, long long n_nodes);
+    const SceneHost &scene(int i) const { return scenes_[i]; }
+    int nscenes() const { return (int)scenes_.size(); }
+    long long total_points() const;
+    float last_step_ms() const { return last_step_ms_; }
+    void clock(int scene, double *t, int *step) const;
+
+    int tx() const { return tx_; }
+    int ty() const { return ty_; }
+    int set_tile_shape(int tx, int ty, int margin);
+
+private:
+    int fail(int code, const std::string &msg);
+    int hipfail(hipError_t e, const char *what);
+    int build_scene_dev(int i, SceneDev &d) const;
+    int push_scenes();
+    int push_clocks();
+    int pull_clocks();
+    int build_tiles();
+    int alloc_bins();
+    StepArgs args() const;
+    int launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed);
+    int zero_grid(int scene, int buf);
+    int record(int cls, bool timed, hipError_t (*)(void *), void *);
+    void timed_begin(bool timed);
+    void timed_end(bool timed, int cls);
+    int collect_timings();
+    void invalidate_graphs();
+    int frac_bits_for(const icempm_simparams &P, int &origin) const;
+
+    int device_ = 0, precision_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::vector<SceneHost> scenes_;
+    bool committed_ = false;
+    int n_cus_ = 256;
+
+    // device memory
+    void *pts_[2] = {nullptr, nullptr};
+    long long pitch_ = 0;          // elements per SoA line
+    long long n_pts_space_ = 0;    // point index space (sum of scene capacities)
+    void *grid_mem_ = nullptr;
+    double *ind_force_mem_ = nullptr;
+    SceneDev *d_scenes_ = nullptr;
+    SceneClock *d_clocks_ = nullptr;
+    unsigned *d_err_ = nullptr;
+    NodeChunk *d_chunks_ = nullptr;
+    int n_chunks_ = 0;
+
+    // binning
+    BinWork bins_{};
+    void *bin_mem_ = nullptr;
+    bool bins_valid_ = false;      // cell_start/tiles describe the current point order
+    void *tile_mem_ = nullptr;
+    TileItem *d_tiles_ = nullptr;
+    int *d_n_tiles_ = nullptr;
+    long long max_tiles_ = 0;
+    int tx_ = 4, ty_ = 32, margin_ = 1;
+    std::vector<SceneDev> host_dev_;
+
+    // timings
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+    std::vector<int> ev_cls_;
+    size_t ev_used_ = 0;
+    double cls_ms_[kNumCls] = {0};
+    long long cls_cnt_[kNumCls] = {0};
+    float last_step_ms_ = 0;
+    hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+
+    // graphs keyed by (nsub, start_buf, fused)
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_;
+};
+
+}  // namespace icempm

@@ -1,0 +1,965 @@
+// engine.hip -- the substep pipeline on one device (host code).
+//
+// Substep sequence of icy::Model::Step (model.cpp:38-54):
+//   t += dt; indenter_x = x0 + t*V; reset_grid; p2g; update_nodes(t); g2p(recordPQ)
+// Binned pipeline, two accumulation grids A/B per scene:
+//   clock; [P2G -> A]; update(A, clear B); fused G2P(A)+P2G(->B); clock; update(B, clear A); ...
+// so each substep after the first reads the points once and writes them once.
+// Unbinned fallback (before the first sort): zero A; P2G direct; update(A); G2P direct.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "icempm_host.h"
+
+namespace icempm {
+
+namespace {
+
+inline long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+int bits_for(long long n)   // smallest b with n <= 2^b
+{
+    int b = 0;
+    while ((1LL << b) < n) b++;
+    return b;
+}
+
+}  // namespace
+
+Engine::Engine(int device, int precision) : device_(device), precision_(precision)
+{
+    hipSetDevice(device_);
+    hipStreamCreate(&stream_);
+    hipEventCreate(&ev_a_);
+    hipEventCreate(&ev_b_);
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device_) == hipSuccess && n > 0) n_cus_ = n;
+}
+
+Engine::~Engine()
+{
+    hipSetDevice(device_);
+    if (stream_) hipStreamSynchronize(stream_);
+    invalidate_graphs();
+    for (auto &e : ev_pool_) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    if (ev_a_) hipEventDestroy(ev_a_);
+    if (ev_b_) hipEventDestroy(ev_b_);
+    void *bufs[] = {pts_[0], pts_[1], grid_mem_, ind_force_mem_, d_scenes_, d_clocks_, d_err_, d_chunks_,
+                    bin_mem_, tile_mem_};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+int Engine::fail(int code, const std::string &msg)
+{
+    set_last_error(msg);
+    return code;
+}
+
+int Engine::hipfail(hipError_t e, const char *what)
+{
+    set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return ICEMPM_ERR_HIP;
+}
+
+#define HIPCHK(expr)                                           \
+    do {                                                       \
+        hipError_t _e = (expr);                                \
+        if (_e != hipSuccess) return hipfail(_e, #expr);       \
+    } while (0)
+
+#define RC(expr)                                               \
+    do {                                                       \
+        int _rc = (expr);                                      \
+        if (_rc != ICEMPM_OK) return _rc;                      \
+    } while (0)
+
+long long Engine::total_points() const
+{
+    long long n = 0;
+    for (auto &s : scenes_) n += s.n_pts;
+    return n;
+}
+
+void Engine::clock(int scene, double *t, int *step) const
+{
+    *t = scenes_[scene].sim_time;
+    *step = scenes_[scene].sim_step;
+}
+
+int Engine::frac_bits_for(const icempm_simparams &P, int &origin) const
+{
+    const int halo = P.GridHaloSize;
+    const int ncols = (scenes_.empty() ? P.GridXTotal : P.GridXTotal) + 2 * halo;
+    origin = halo + 2;
+    const long long span = std::max<long long>(ncols + 4, P.GridY + 4);
+    return std::min(22, 32 - bits_for(span));
+}
+
+int Engine::add_scene(const icempm_simparams *prms, int cap_pts)
+{
+    if (cap_pts <= 0) return fail(ICEMPM_ERR_ARGUMENT, "add_scene: capacity must be positive");
+    SceneHost s;
+    s.cap_pts = cap_pts;
+    if (prms) {
+        s.prms = *prms;
+        s.have_prms = true;
+        s.sim_time = prms->SimulationTime;
+        s.sim_step = prms->SimulationStep;
+    }
+    scenes_.push_back(s);
+    committed_ = false;
+    return (int)scenes_.size() - 1;
+}
+
+int Engine::set_params(int scene, const icempm_simparams &P)
+{
+    if (scene < 0 || scene >= nscenes()) return fail(ICEMPM_ERR_ARGUMENT, "set_params: bad scene");
+    if (P.GridXTotal <= 0 || P.GridY < 4 || P.GridHaloSize < 0 || !(P.cellsize > 0) || !(P.InitialTimeStep > 0) ||
+        P.n_indenter_subdivisions <= 0 || P.n_indenter_subdivisions > kMaxIndenterSub)
+        return fail(ICEMPM_ERR_ARGUMENT, "update_constants: parameters not derived or out of range");
+    SceneHost &s = scenes_[scene];
+    const bool geometry_changed = !s.have_prms || s.prms.GridXTotal != P.GridXTotal || s.prms.GridY != P.GridY ||
+                                  s.prms.GridHaloSize != P.GridHaloSize ||
+                                  s.prms.n_indenter_subdivisions != P.n_indenter_subdivisions;
+    const bool position_map_changed =
+        precision_ == ICEMPM_PRECISION_MIXED && s.have_prms && s.prms.cellsize != P.cellsize;
+    if (position_map_changed && s.n_pts > 0)
+        return fail(ICEMPM_ERR_STATE, "update_constants: the mixed form cannot change the cell size of uploaded points");
+    s.prms = P;
+    s.have_prms = true;
+    s.sim_time = P.SimulationTime;
+    s.sim_step = P.SimulationStep;
+    invalidate_graphs();
+    if (geometry_changed && committed_) {
+        // grids depend on the geometry: rebuild everything but keep the points
+        committed_ = false;
+        bins_valid_ = false;
+    }
+    if (committed_) {
+        RC(push_scenes());
+        RC(push_clocks());
+        HIPCHK(hipMemsetAsync(d_err_, 0, sizeof(unsigned) * nscenes(), stream_));
+    }
+    return ICEMPM_OK;
+}
+
+int Engine::set_grid_span(int scene, int xoff, int gx)
+{
+    if (scene < 0 || scene >= nscenes() || !scenes_[scene].have_prms)
+        return fail(ICEMPM_ERR_STATE, "set_grid_span: parameters not set");
+    SceneHost &s = scenes_[scene];
+    if (xoff != 0 || gx != s.prms.GridXTotal)
+        return fail(ICEMPM_ERR_ARGUMENT,
+                    "set_grid_span: one partition spans the whole grid on one device (GridX_offset 0, "
+                    "GridX_partition = GridXTotal)");
+    return ICEMPM_OK;
+}
+
+int Engine::build_scene_dev(int i, SceneDev &d) const
+{
+    const SceneHost &s = scenes_[i];
+    const icempm_simparams &P = s.prms;
+    std::memset(&d, 0, sizeof d);
+    SceneConsts &c = d.c;
+    c.gridX = P.GridXTotal;
+    c.gridY = P.GridY;
+    c.halo = P.GridHaloSize;
+    c.ncols = c.gridX + 2 * c.halo;
+    c.xoff = 0;
+    c.gridXTotal = P.GridXTotal;
+    c.n_ind_sub = P.n_indenter_subdivisions;
+    c.update_every = std::max(1, P.UpdateEveryNthStep);
+    int origin = 0;
+    c.frac_bits = frac_bits_for(P, origin);
+    c.origin = origin;
+    c.origin_y = 2;
+    c.h = P.cellsize;
+    c.hinv = P.cellsize_inv;
+    c.dt = P.InitialTimeStep;
+    c.dt_vol_Dpinv = P.dt_vol_Dpinv;
+    c.dt_gravity = P.dt_Gravity;
+    c.vmax = P.vmax;
+    c.vmax_sq = P.vmax_squared;
+    c.pmass = P.ParticleMass;
+    c.wmass = P.WaterParticleMass;
+    c.mu = P.mu;
+    c.kappa = P.kappa;
+    c.ics = P.IceCompressiveStrength;
+    c.its = P.IceTensileStrength;
+    c.iss = P.IceShearStrength;
+    c.its2 = P.IceTensileStrength2;
+    c.beta = P.NACC_beta;
+    c.grain_var = P.GrainVariability;
+    c.tan_phi = P.DP_tan_phi;
+    c.dp_threshold_p = P.DP_threshold_p;
+    c.ind_rsq = P.IndRSq;
+    c.ind_vel = P.IndVelocity;
+    c.ind_x0 = P.indenter_x_initial;
+    c.ind_y = P.indenter_y;
+    d.pt_off = s.pt_off;
+    d.n_pts = s.n_pts;
+    d.cap_pts = s.cap_pts;
+    d.grid[0] = s.grid[0];
+    d.grid[1] = s.grid[1];
+    d.gpitch = s.grid_elems;
+    d.cell_off = s.cells_off;
+    d.ncells = c.ncols * c.gridY;
+    d.n_tiles_x = (c.ncols + tx_ - 1) / tx_;
+    d.n_tiles_y = (c.gridY + ty_ - 1) / ty_;
+    d.ind_force = s.ind_force;
+    return ICEMPM_OK;
+}
+
+int Engine::push_scenes()
+{
+    host_dev_.resize(nscenes());
+    for (int i = 0; i < nscenes(); i++) build_scene_dev(i, host_dev_[i]);
+    HIPCHK(hipMemcpyAsync(d_scenes_, host_dev_.data(), sizeof(SceneDev) * nscenes(), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::push_clocks()
+{
+    std::vector<SceneClock> ck(nscenes());
+    for (int i = 0; i < nscenes(); i++) {
+        const SceneHost &s = scenes_[i];
+        ck[i].t = s.sim_time;
+        ck[i].ind_x = s.prms.indenter_x;
+        ck[i].ind_y = s.prms.indenter_y;
+        ck[i].step = s.sim_step;
+        ck[i].next_step = s.sim_step;
+    }
+    HIPCHK(hipMemcpyAsync(d_clocks_, ck.data(), sizeof(SceneClock) * nscenes(), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::pull_clocks()
+{
+    std::vector<SceneClock> ck(nscenes());
+    HIPCHK(hipMemcpyAsync(ck.data(), d_clocks_, sizeof(SceneClock) * nscenes(), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < nscenes(); i++) {
+        scenes_[i].sim_time = ck[i].t;
+        scenes_[i].sim_step = ck[i].next_step;
+    }
+    return ICEMPM_OK;
+}
+
+void Engine::invalidate_graphs()
+{
+    for (auto &g : graphs_) hipGraphExecDestroy(g.second);
+    graphs_.clear();
+}
+
+// Allocates all device memory for the current scene list.  Points already on
+// the device are preserved when only the grids change.
+int Engine::commit()
+{
+    hipSetDevice(device_);
+    if (scenes_.empty()) return fail(ICEMPM_ERR_STATE, "commit: no scene");
+    for (auto &s : scenes_)
+        if (!s.have_prms) return fail(ICEMPM_ERR_STATE, "commit: parameters of a scene are not set");
+    invalidate_graphs();
+    HIPCHK(hipStreamSynchronize(stream_));
+    const int eb = elem_bytes();
+
+    // points (kept if the layout is unchanged)
+    long long off = 0;
+    std::vector<long long> offs(nscenes());
+    for (int i = 0; i < nscenes(); i++) {
+        offs[i] = off;
+        off += round_up(scenes_[i].cap_pts, 256);
+    }
+    const long long space = off;
+    bool points_layout_same = pts_[0] && space == n_pts_space_;
+    for (int i = 0; i < nscenes() && points_layout_same; i++) points_layout_same = offs[i] == scenes_[i].pt_off;
+    if (!points_layout_same) {
+        for (int b = 0; b < 2; b++)
+            if (pts_[b]) { hipFree(pts_[b]); pts_[b] = nullptr; }
+        n_pts_space_ = space;
+        pitch_ = round_up(space, 256);
+        for (int b = 0; b < 2; b++) {
+            HIPCHK(hipMalloc(&pts_[b], (size_t)kNumPtsLines * pitch_ * eb));
+            HIPCHK(hipMemsetAsync(pts_[b], 0, (size_t)kNumPtsLines * pitch_ * eb, stream_));
+        }
+        for (int i = 0; i < nscenes(); i++) { scenes_[i].pt_off = offs[i]; scenes_[i].n_pts = 0; }
+    }
+
+    // grids: [buffer 0 of all scenes][buffer 1 of all scenes], 24 bytes per node each
+    if (grid_mem_) { hipFree(grid_mem_); grid_mem_ = nullptr; }
+    long long gelems = 0, cells = 0, forces = 0;
+    std::vector<long long> goff(nscenes()), coff(nscenes()), foff(nscenes());
+    const long long lines = precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3;
+    for (int i = 0; i < nscenes(); i++) {
+        const icempm_simparams &P = scenes_[i].prms;
+        const long long ncols = P.GridXTotal + 2LL * P.GridHaloSize;
+        const long long nodes = ncols * P.GridY;
+        scenes_[i].grid_elems = round_up(nodes, 64);
+        goff[i] = gelems;
+        gelems += lines * scenes_[i].grid_elems;
+        coff[i] = cells;
+        cells += nodes + 1;
+        foff[i] = forces;
+        forces += 2LL * P.n_indenter_subdivisions;
+    }
+    const size_t buf_bytes = (size_t)gelems * eb;
+    HIPCHK(hipMalloc(&grid_mem_, 2 * buf_bytes));
+    HIPCHK(hipMemsetAsync(grid_mem_, 0, 2 * buf_bytes, stream_));
+    if (ind_force_mem_) hipFree(ind_force_mem_);
+    HIPCHK(hipMalloc(&ind_force_mem_, sizeof(double) * std::max<long long>(forces, 1)));
+    HIPCHK(hipMemsetAsync(ind_force_mem_, 0, sizeof(double) * std::max<long long>(forces, 1), stream_));
+    for (int i = 0; i < nscenes(); i++) {
+        scenes_[i].grid[0] = (char *)grid_mem_ + goff[i] * eb;
+        scenes_[i].grid[1] = (char *)grid_mem_ + buf_bytes + goff[i] * eb;
+        scenes_[i].cells_off = coff[i];
+        scenes_[i].ind_force = ind_force_mem_ + foff[i];
+        scenes_[i].clean[0] = scenes_[i].clean[1] = true;
+    }
+
+    // tables
+    if (d_scenes_) hipFree(d_scenes_);
+    if (d_clocks_) hipFree(d_clocks_);
+    if (d_err_) hipFree(d_err_);
+    HIPCHK(hipMalloc(&d_scenes_, sizeof(SceneDev) * nscenes()));
+    HIPCHK(hipMalloc(&d_clocks_, sizeof(SceneClock) * nscenes()));
+    HIPCHK(hipMalloc(&d_err_, sizeof(unsigned) * nscenes()));
+    HIPCHK(hipMemsetAsync(d_err_, 0, sizeof(unsigned) * nscenes(), stream_));
+
+    std::vector<NodeChunk> chunks;
+    for (int i = 0; i < nscenes(); i++) {
+        const icempm_simparams &P = scenes_[i].prms;
+        const int nodes = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY;
+        for (int n0 = 0; n0 < nodes; n0 += 1024) chunks.push_back({i, n0, std::min(1024, nodes - n0), 0});
+    }
+    if (d_chunks_) hipFree(d_chunks_);
+    n_chunks_ = (int)chunks.size();
+    HIPCHK(hipMalloc(&d_chunks_, sizeof(NodeChunk) * chunks.size()));
+    HIPCHK(hipMemcpy(d_chunks_, chunks.data(), sizeof(NodeChunk) * chunks.size(), hipMemcpyHostToDevice));
+
+    bins_.n_cells_total = cells;
+    bins_.n_pts_total = n_pts_space_;
+    RC(alloc_bins());
+    committed_ = true;
+    bins_valid_ = false;
+    RC(push_scenes());
+    RC(push_clocks());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::alloc_bins()
+{
+    if (bin_mem_) { hipFree(bin_mem_); bin_mem_ = nullptr; }
+    const long long np = std::max<long long>(n_pts_space_, 1), nc = std::max<long long>(bins_.n_cells_total, 1);
+    const long long nscan = std::max(np, nc) / 1000 + 64;
+    const long long n_int = 3 * np + 3 * nc + nscan + nscenes() + 16;
+    HIPCHK(hipMalloc(&bin_mem_, sizeof(int) * n_int));
+    int *p = (int *)bin_mem_;
+    bins_.keys = p; p += np;
+    bins_.perm = p; p += np;
+    bins_.sorted_keys = p; p += np;
+    bins_.cell_count = p; p += nc;
+    bins_.cell_fill = p; p += nc;
+    bins_.cell_start = p; p += nc;
+    bins_.scan_tmp = p; p += nscan;
+    bins_.scene_count = p;
+    HIPCHK(hipMemsetAsync(bin_mem_, 0, sizeof(int) * n_int, stream_));
+    // tile table for the current shape
+    if (tile_mem_) { hipFree(tile_mem_); tile_mem_ = nullptr; }
+    long long mt = 0;
+    for (auto &s : scenes_) {
+        const long long ncols = s.prms.GridXTotal + 2LL * s.prms.GridHaloSize;
+        mt += ((ncols + tx_ - 1) / tx_) * ((s.prms.GridY + ty_ - 1) / ty_);
+    }
+    max_tiles_ = mt;
+    const size_t tbytes = sizeof(TileItem) * mt + sizeof(int) * (3 * (mt + nscenes() + 1) + mt / 1000 + 64) + 64;
+    HIPCHK(hipMalloc(&tile_mem_, tbytes));
+    HIPCHK(hipMemsetAsync(tile_mem_, 0, tbytes, stream_));
+    d_tiles_ = (TileItem *)tile_mem_;
+    d_n_tiles_ = (int *)((char *)tile_mem_ + tbytes - 64);
+    return ICEMPM_OK;
+}
+
+int Engine::ensure_committed()
+{
+    if (committed_) return ICEMPM_OK;
+    return commit();
+}
+
+StepArgs Engine::args() const
+{
+    StepArgs a{};
+    a.scenes = d_scenes_;
+    a.clocks = d_clocks_;
+    a.pts = pts_[0];
+    a.pitch = pitch_;
+    a.cell_start = bins_.cell_start;
+    a.tiles = d_tiles_;
+    a.n_tiles = d_n_tiles_;
+    a.chunks = d_chunks_;
+    a.n_chunks = n_chunks_;
+    a.nscenes = nscenes();
+    a.err = d_err_;
+    a.tx = tx_;
+    a.ty = ty_;
+    a.margin = margin_;
+    a.win_w = tx_ + 2 + 2 * margin_;
+    a.win_h = ty_ + 2 + 2 * margin_;
+    return a;
+}
+
+int Engine::set_tile_shape(int tx, int ty, int margin)
+{
+    if (tx < 1 || tx > 32 || ty < 1 || ty > 1024 || margin < 0 || margin > 4)
+        return fail(ICEMPM_ERR_ARGUMENT, "tile shape out of range");
+    const size_t lds = (size_t)(tx + 2 + 2 * margin) * (ty + 2 + 2 * margin) * 5 * sizeof(double);
+    if (lds > 64 * 1024) return fail(ICEMPM_ERR_ARGUMENT, "tile window exceeds the LDS budget");
+    if (tx == tx_ && ty == ty_ && margin == margin_) return ICEMPM_OK;
+    tx_ = tx;
+    ty_ = ty;
+    margin_ = margin;
+    invalidate_graphs();
+    if (!committed_) return ICEMPM_OK;
+    // the tile table depends on the shape; cell_start does not
+    if (tile_mem_) { hipFree(tile_mem_); tile_mem_ = nullptr; }
+    long long mt = 0;
+    for (auto &s : scenes_) {
+        const long long ncols = s.prms.GridXTotal + 2LL * s.prms.GridHaloSize;
+        mt += ((ncols + tx_ - 1) / tx_) * ((s.prms.GridY + ty_ - 1) / ty_);
+    }
+    max_tiles_ = mt;
+    const size_t tbytes = sizeof(TileItem) * mt + sizeof(int) * (3 * (mt + nscenes() + 1) + mt / 1000 + 64) + 64;
+    HIPCHK(hipMalloc(&tile_mem_, tbytes));
+    HIPCHK(hipMemsetAsync(tile_mem_, 0, tbytes, stream_));
+    d_tiles_ = (TileItem *)tile_mem_;
+    d_n_tiles_ = (int *)((char *)tile_mem_ + tbytes - 64);
+    RC(push_scenes());
+    if (bins_valid_) RC(build_tiles());
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// points in and out
+// ---------------------------------------------------------------------------
+int Engine::upload(int scene, const double *hssoa, unsigned capacity, int offset, int n)
+{
+    RC(ensure_committed());
+    SceneHost &s = scenes_[scene];
+    if (n < 0 || n > s.cap_pts) return fail(ICEMPM_ERR_CAPACITY, "transfer_points_from_soa_to_device: capacity");
+    if (offset < 0 || (unsigned)(offset + n) > capacity)
+        return fail(ICEMPM_ERR_CAPACITY, "transfer_points_from_soa_to_device: HSSOA capacity");
+    if (precision_ == ICEMPM_PRECISION_FP64) {
+        for (int line = 0; line < kNumPtsLines; line++)
+            HIPCHK(hipMemcpyAsync((double *)pts_[0] + line * pitch_ + s.pt_off, hssoa + (size_t)line * capacity + offset,
+                                  sizeof(double) * n, hipMemcpyHostToDevice, stream_));
+    } else {
+        const SceneDev &d = host_dev_[scene];
+        const double scale = std::ldexp(1.0, d.c.frac_bits);
+        const double lim = 4294967295.0;
+        std::vector<uint32_t> buf((size_t)kNumPtsLines * n);
+        auto L = [&](int line, int i) { return hssoa[(size_t)line * capacity + offset + i]; };
+        for (int i = 0; i < n; i++) {
+            long long ud;
+            std::memcpy(&ud, hssoa + offset + i, 8);
+            buf[(size_t)0 * n + i] = (uint32_t)ud;
+            const int org[2] = {d.c.origin, d.c.origin_y};
+            for (int k = 0; k < 2; k++) {
+                double q = std::nearbyint((L(kLinePosX + k, i) * d.c.hinv + org[k]) * scale);
+                q = std::min(std::max(q, 0.0), lim);
+                buf[(size_t)(kLinePosX + k) * n + i] = (uint32_t)q;
+            }
+            auto putf = [&](int line, double v) {
+                const float f = (float)v;
+                std::memcpy(&buf[(size_t)line * n + i], &f, 4);
+            };
+            putf(kLineP, L(kLineP, i));
+            putf(kLineQ, L(kLineQ, i));
+            putf(kLineJp, L(kLineJp, i));
+            putf(kLineVelX, L(kLineVelX, i));
+            putf(kLineVelX + 1, L(kLineVelX + 1, i));
+            putf(kLineFe00, L(kLineFe00, i) - 1.0);
+            putf(kLineFe00 + 1, L(kLineFe00 + 1, i));
+            putf(kLineFe00 + 2, L(kLineFe00 + 2, i));
+            putf(kLineFe00 + 3, L(kLineFe00 + 3, i) - 1.0);
+            for (int k = 0; k < 4; k++) putf(kLineBp00 + k, L(kLineBp00 + k, i));
+        }
+        for (int line = 0; line < kNumPtsLines; line++)
+            HIPCHK(hipMemcpyAsync((uint32_t *)pts_[0] + line * pitch_ + s.pt_off, buf.data() + (size_t)line * n,
+                                  sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    s.n_pts = n;
+    bins_valid_ = false;
+    invalidate_graphs();
+    RC(push_scenes());
+    return ICEMPM_OK;
+}
+
+int Engine::download(int scene, double *hssoa, unsigned capacity, int offset, int *n_out)
+{
+    RC(ensure_committed());
+    SceneHost &s = scenes_[scene];
+    const int n = s.n_pts;
+    if (offset < 0 || (unsigned)(offset + n) > capacity)
+        return fail(ICEMPM_ERR_CAPACITY, "transfer_from_device() HSSOA capacity");
+    if (precision_ == ICEMPM_PRECISION_FP64) {
+        for (int line = 0; line < kNumPtsLines; line++)
+            HIPCHK(hipMemcpyAsync(hssoa + (size_t)line * capacity + offset, (double *)pts_[0] + line * pitch_ + s.pt_off,
+                                  sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    } else {
+        std::vector<uint32_t> buf((size_t)kNumPtsLines * std::max(n, 1));
+        for (int line = 0; line < kNumPtsLines; line++)
+            HIPCHK(hipMemcpyAsync(buf.data() + (size_t)line * n, (uint32_t *)pts_[0] + line * pitch_ + s.pt_off,
+                                  sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        const SceneDev &d = host_dev_[scene];
+        const double inv = std::ldexp(1.0, -d.c.frac_bits);
+        for (int i = 0; i < n; i++) {
+            auto getf = [&](int line) {
+                float f;
+                std::memcpy(&f, &buf[(size_t)line * n + i], 4);
+                return (double)f;
+            };
+            auto put = [&](int line, double v) { hssoa[(size_t)line * capacity + offset + i] = v; };
+            const long long ud = (long long)buf[i];
+            std::memcpy(hssoa + offset + i, &ud, 8);
+            const int org[2] = {d.c.origin, d.c.origin_y};
+            for (int k = 0; k < 2; k++)
+                put(kLinePosX + k, ((double)buf[(size_t)(kLinePosX + k) * n + i] * inv - org[k]) * d.c.h);
+            put(kLineP, getf(kLineP));
+            put(kLineQ, getf(kLineQ));
+            put(kLineJp, getf(kLineJp));
+            put(kLineVelX, getf(kLineVelX));
+            put(kLineVelX + 1, getf(kLineVelX + 1));
+            put(kLineFe00, getf(kLineFe00) + 1.0);
+            put(kLineFe00 + 1, getf(kLineFe00 + 1));
+            put(kLineFe00 + 2, getf(kLineFe00 + 2));
+            put(kLineFe00 + 3, getf(kLineFe00 + 3) + 1.0);
+            for (int k = 0; k < 4; k++) put(kLineBp00 + k, getf(kLineBp00 + k));
+        }
+    }
+    if (n_out) *n_out = n;
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// timing helpers
+// ---------------------------------------------------------------------------
+void Engine::timed_begin(bool timed)
+{
+    if (!timed) return;
+    if (ev_used_ == ev_pool_.size()) {
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        ev_pool_.push_back({a, b});
+        ev_cls_.push_back(0);
+    }
+    hipEventRecord(ev_pool_[ev_used_].first, stream_);
+}
+
+void Engine::timed_end(bool timed, int cls)
+{
+    if (!timed) return;
+    hipEventRecord(ev_pool_[ev_used_].second, stream_);
+    ev_cls_[ev_used_] = cls;
+    ev_used_++;
+}
+
+int Engine::collect_timings()
+{
+    if (ev_used_ == 0) return ICEMPM_OK;
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < ev_used_; i++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev_pool_[i].first, ev_pool_[i].second));
+        cls_ms_[ev_cls_[i]] += ms;
+        cls_cnt_[ev_cls_[i]]++;
+    }
+    ev_used_ = 0;
+    return ICEMPM_OK;
+}
+
+int Engine::get_kernel_stats(double *ms, long long *count, int n)
+{
+    RC(collect_timings());
+    for (int k = 0; k < n && k < kNumCls; k++) {
+        if (ms) ms[k] = cls_ms_[k];
+        if (count) count[k] = cls_cnt_[k];
+    }
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phases
+// ---------------------------------------------------------------------------
+int Engine::zero_grid(int scene, int buf)
+{
+    const SceneHost &s = scenes_[scene];
+    HIPCHK(launch_fill_zero(s.grid[buf], (size_t)s.grid_elems * (precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3) *
+                                             elem_bytes(), stream_));
+    scenes_[scene].clean[buf] = true;
+    return ICEMPM_OK;
+}
+
+#define LAUNCH_PREC(fn, ...) \
+    (precision_ == ICEMPM_PRECISION_MIXED ? mixed::fn(__VA_ARGS__) : fp64::fn(__VA_ARGS__))
+
+int Engine::reset_grid(int scene)
+{
+    RC(ensure_committed());
+    return zero_grid(scene, 0);
+}
+
+int Engine::p2g(int scene)
+{
+    RC(ensure_committed());
+    const StepArgs a = args();
+    if (bins_valid_ && nscenes() == 1) {
+        const int nb = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
+        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, 0, 0, 0, nb, stream_));
+    } else {
+        HIPCHK(LAUNCH_PREC(launch_p2g_direct, a, scene, scenes_[scene].n_pts, 0, stream_));
+    }
+    scenes_[scene].clean[0] = false;
+    return ICEMPM_OK;
+}
+
+int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
+{
+    RC(ensure_committed());
+    SceneClock ck{};
+    ck.t = t;
+    ck.ind_x = ind_x;
+    ck.ind_y = ind_y;
+    ck.step = scenes_[scene].sim_step;
+    ck.next_step = scenes_[scene].sim_step;
+    HIPCHK(hipMemcpyAsync(d_clocks_ + scene, &ck, sizeof ck, hipMemcpyHostToDevice, stream_));
+    StepArgs a = args();
+    // only this scene's chunks: the chunk table is scene-major
+    const NodeChunk *first = nullptr;
+    int nchunks = 0;
+    {
+        long long k = 0;
+        for (int i = 0; i < nscenes(); i++) {
+            const icempm_simparams &P = scenes_[i].prms;
+            const long long nodes = (long long)(P.GridXTotal + 2 * P.GridHaloSize) * P.GridY;
+            const int nc = (int)((nodes + 1023) / 1024);
+            if (i == scene) { first = d_chunks_ + k; nchunks = nc; }
+            k += nc;
+        }
+    }
+    a.chunks = first;
+    a.n_chunks = nchunks;
+    HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, std::min(nchunks, n_cus_ * 8), stream_));
+    HIPCHK(hipStreamSynchronize(stream_));   // ck lives on this stack frame
+    return ICEMPM_OK;
+}
+
+int Engine::g2p(int scene, int recordPQ)
+{
+    RC(ensure_committed());
+    const StepArgs a = args();
+    if (bins_valid_ && nscenes() == 1) {
+        const int nb = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
+        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, 0, 0, recordPQ ? 1 : 0, nb, stream_));
+    } else {
+        HIPCHK(LAUNCH_PREC(launch_g2p_direct, a, scene, scenes_[scene].n_pts, 0, recordPQ ? 1 : 0, stream_));
+    }
+    scenes_[scene].sim_step++;
+    return ICEMPM_OK;
+}
+
+int Engine::reset_indenter_force()
+{
+    RC(ensure_committed());
+    long long forces = 0;
+    for (auto &s : scenes_) forces += 2LL * s.prms.n_indenter_subdivisions;
+    HIPCHK(launch_fill_zero(ind_force_mem_, sizeof(double) * forces, stream_));
+    RC(collect_timings());
+    for (int k = 0; k < kNumCls; k++) { cls_ms_[k] = 0; cls_cnt_[k] = 0; }
+    return ICEMPM_OK;
+}
+
+int Engine::synchronize()
+{
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::get_indenter_force(int scene, double *out, int n)
+{
+    RC(ensure_committed());
+    const SceneHost &s = scenes_[scene];
+    const int m = std::min(n, 2 * s.prms.n_indenter_subdivisions);
+    HIPCHK(hipMemcpyAsync(out, s.ind_force, sizeof(double) * m, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::get_error_code(uint8_t *code)
+{
+    RC(ensure_committed());
+    std::vector<unsigned> e(nscenes());
+    HIPCHK(hipMemcpyAsync(e.data(), d_err_, sizeof(unsigned) * nscenes(), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    unsigned m = 0;
+    for (unsigned v : e) m = std::max(m, v);
+    *code = (uint8_t)m;
+    return ICEMPM_OK;
+}
+
+int Engine::get_grid(int scene, int buf, double *out, long long n_nodes)
+{
+    RC(ensure_committed());
+    if (buf < 0 || buf > 1) return fail(ICEMPM_ERR_ARGUMENT, "get_grid: buffer 0 or 1");
+    const SceneHost &s = scenes_[scene];
+    const long long nodes = (long long)(s.prms.GridXTotal + 2 * s.prms.GridHaloSize) * s.prms.GridY;
+    if (n_nodes != nodes) return fail(ICEMPM_ERR_ARGUMENT, "get_grid: node count mismatch");
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (precision_ == ICEMPM_PRECISION_FP64) {
+        for (int line = 0; line < 3; line++)
+            HIPCHK(hipMemcpy(out + line * nodes, (double *)s.grid[buf] + line * s.grid_elems, sizeof(double) * nodes,
+                             hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> hi(nodes), lo(nodes);
+        for (int line = 0; line < 3; line++) {
+            HIPCHK(hipMemcpy(hi.data(), (float *)s.grid[buf] + (2 * line) * s.grid_elems, sizeof(float) * nodes,
+                             hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(lo.data(), (float *)s.grid[buf] + (2 * line + 1) * s.grid_elems, sizeof(float) * nodes,
+                             hipMemcpyDeviceToHost));
+            for (long long i = 0; i < nodes; i++) out[line * nodes + i] = (double)hi[i] + (double)lo[i];
+        }
+    }
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// binning
+// ---------------------------------------------------------------------------
+int Engine::build_tiles()
+{
+    const StepArgs a = args();
+    HIPCHK(launch_build_tiles(a, host_dev_.data(), nscenes(), bins_, d_tiles_, d_n_tiles_, max_tiles_, stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::sort()
+{
+    RC(ensure_committed());
+    invalidate_graphs();
+    StepArgs a = args();
+    timed_begin(false);
+    HIPCHK(launch_sort(a, host_dev_.data(), nscenes(), bins_, elem_bytes(), pts_[0], pts_[1], stream_));
+    std::swap(pts_[0], pts_[1]);
+    std::vector<int> cnt(nscenes());
+    HIPCHK(hipMemcpyAsync(cnt.data(), bins_.scene_count, sizeof(int) * nscenes(), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < nscenes(); i++) scenes_[i].n_pts = scenes_[i].n_pts > 0 ? cnt[i] : 0;
+    RC(push_scenes());
+    RC(build_tiles());
+    bins_valid_ = true;
+    return ICEMPM_OK;
+}
+
+int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts, int n_cells)
+{
+    if (!bins_valid_) return fail(ICEMPM_ERR_STATE, "get_bins: no sort since the last upload");
+    const SceneHost &s = scenes_[scene];
+    const int m = std::min(n, s.n_pts);
+    if (keys) HIPCHK(hipMemcpy(keys, bins_.sorted_keys + s.pt_off, sizeof(int) * m, hipMemcpyDeviceToHost));
+    if (perm) {
+        HIPCHK(hipMemcpy(perm, bins_.perm + s.pt_off, sizeof(int) * m, hipMemcpyDeviceToHost));
+        for (int i = 0; i < m; i++) perm[i] -= (int32_t)s.pt_off;
+    }
+    if (cell_counts) {
+        const int nc = (s.prms.GridXTotal + 2 * s.prms.GridHaloSize) * s.prms.GridY;
+        std::vector<int> st(nc + 1);
+        HIPCHK(hipMemcpy(st.data(), bins_.cell_start + s.cells_off, sizeof(int) * (nc + 1), hipMemcpyDeviceToHost));
+        for (int i = 0; i < std::min(nc, n_cells); i++) cell_counts[i] = st[i + 1] - st[i];
+    }
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// frame stepping
+// ---------------------------------------------------------------------------
+int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed)
+{
+    const StepArgs a = args();
+    const int nb_tiles = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
+    const int nb_upd = std::min(n_chunks_, n_cus_ * 8);
+    int b = start_buf;
+    for (int k = 0; k < nsub; k++) {
+        timed_begin(timed);
+        HIPCHK(launch_clock(a, k == 0, stream_));
+        timed_end(timed, kClsClock);
+        if (direct) {
+            timed_begin(timed);
+            for (int i = 0; i < nscenes(); i++) {
+                const SceneHost &s = scenes_[i];
+                HIPCHK(launch_fill_zero(s.grid[0], (size_t)s.grid_elems *
+                                                       (precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3) * elem_bytes(),
+                                        stream_));
+            }
+            timed_end(timed, kClsZero);
+            timed_begin(timed);
+            for (int i = 0; i < nscenes(); i++)
+                HIPCHK(LAUNCH_PREC(launch_p2g_direct, a, i, scenes_[i].n_pts, 0, stream_));
+            timed_end(timed, kClsP2G);
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, nb_upd, stream_));
+            timed_end(timed, kClsUpdate);
+            timed_begin(timed);
+            for (int i = 0; i < nscenes(); i++)
+                HIPCHK(LAUNCH_PREC(launch_g2p_direct, a, i, scenes_[i].n_pts, 0, -1, stream_));
+            timed_end(timed, kClsG2P);
+            continue;
+        }
+        if (k == 0 || !fused) {
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, b, b, -1, nb_tiles, stream_));
+            timed_end(timed, kClsP2G);
+        }
+        timed_begin(timed);
+        HIPCHK(LAUNCH_PREC(launch_update, a, b, 1, nb_upd, stream_));
+        timed_end(timed, kClsUpdate);
+        if (fused && k < nsub - 1) {
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileFused, b, b ^ 1, -1, nb_tiles, stream_));
+            timed_end(timed, kClsFused);
+        } else {
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, b, b, -1, nb_tiles, stream_));
+            timed_end(timed, kClsG2P);
+        }
+        b ^= 1;
+    }
+    return ICEMPM_OK;
+}
+
+int Engine::step(int nsub, int flags)
+{
+    RC(ensure_committed());
+    if (nsub <= 0) return ICEMPM_OK;
+    if (flags & ICEMPM_STEP_RESORT) RC(sort());
+    const bool direct = (flags & ICEMPM_STEP_DIRECT_P2G) || !bins_valid_;
+    const bool fused = !(flags & ICEMPM_STEP_UNFUSED);
+    const bool timed = (flags & ICEMPM_STEP_TIMINGS) != 0;
+    const bool use_graph = (flags & ICEMPM_STEP_GRAPH) && !timed;
+
+    int start_buf = 0;
+    if (!direct) {
+        bool c0 = true, c1 = true;
+        for (auto &s : scenes_) { c0 = c0 && s.clean[0]; c1 = c1 && s.clean[1]; }
+        if (c0) start_buf = 0;
+        else if (c1) start_buf = 1;
+        else {
+            for (int i = 0; i < nscenes(); i++) RC(zero_grid(i, 0));
+            start_buf = 0;
+        }
+    }
+    HIPCHK(hipEventRecord(ev_a_, stream_));
+    if (use_graph) {
+        const auto key = std::make_tuple(nsub, start_buf, (direct ? 2 : 0) | (fused ? 1 : 0));
+        auto it = graphs_.find(key);
+        if (it == graphs_.end()) {
+            HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_substeps(nsub, start_buf, fused, direct, false);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(stream_, &g);
+            if (rc != ICEMPM_OK) { if (g) hipGraphDestroy(g); return rc; }
+            if (ec != hipSuccess) return hipfail(ec, "hipStreamEndCapture");
+            hipGraphExec_t ge = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (ei != hipSuccess) return hipfail(ei, "hipGraphInstantiate");
+            it = graphs_.emplace(key, ge).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, stream_));
+    } else {
+        RC(launch_substeps(nsub, start_buf, fused, direct, timed));
+    }
+    HIPCHK(hipEventRecord(ev_b_, stream_));
+    HIPCHK(hipEventSynchronize(ev_b_));
+    HIPCHK(hipEventElapsedTime(&last_step_ms_, ev_a_, ev_b_));
+    last_step_ms_ /= nsub;
+    if (timed) RC(collect_timings());
+
+    for (auto &s : scenes_) {
+        if (direct) {
+            s.clean[0] = false;
+        } else {
+            const int last = start_buf ^ ((nsub - 1) & 1);
+            s.clean[last] = false;
+            s.clean[last ^ 1] = true;
+        }
+        for (int k = 0; k < nsub; k++) s.sim_time += s.prms.InitialTimeStep;
+        s.sim_step += nsub;
+    }
+    return ICEMPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: time the binned pipeline for each candidate tile shape on a copy
+// of the current state, keep the fastest, restore the state.
+// ---------------------------------------------------------------------------
+int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
+{
+    RC(ensure_committed());
+    if (!bins_valid_) RC(sort());
+    static const int cand[][2] = {{2, 32}, {4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {16, 16}, {2, 64}};
+    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+    const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
+    // backup: points into the sort buffer, clocks and forces on the host
+    HIPCHK(hipMemcpyAsync(pts_[1], pts_[0], pts_bytes, hipMemcpyDeviceToDevice, stream_));
+    std::vector<SceneHost> saved = scenes_;
+    const int tx0 = tx_, ty0 = ty_;
+    float best = 1e30f;
+    int best_i = -1;
+    n_trials = std::max(1, n_trials);
+    for (int c = 0; c < ncand; c++) {
+        if (set_tile_shape(cand[c][0], cand[c][1], margin_) != ICEMPM_OK) {
+            if (c < n_out && out_ms) out_ms[c] = -1;
+            continue;
+        }
+        RC(step(1, 0));   // warm-up
+        HIPCHK(hipEventRecord(ev_a_, stream_));
+        RC(launch_substeps(n_trials, scenes_[0].clean[0] ? 0 : 1, true, false, false));
+        HIPCHK(hipEventRecord(ev_b_, stream_));
+        HIPCHK(hipEventSynchronize(ev_b_));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev_a_, ev_b_));
+        ms /= n_trials;
+        if (c < n_out && out_ms) out_ms[c] = ms;
+        if (ms < best) { best = ms; best_i = c; }
+        // restore points and grids
+        HIPCHK(hipMemcpyAsync(pts_[0], pts_[1], pts_bytes, hipMemcpyDeviceToDevice, stream_));
+        for (int i = 0; i < nscenes(); i++) { RC(zero_grid(i, 0)); RC(zero_grid(i, 1)); }
+    }
+    for (int i = 0; i < nscenes(); i++) {
+        scenes_[i].sim_time = saved[i].sim_time;
+        scenes_[i].sim_step = saved[i].sim_step;
+    }
+    RC(push_clocks());
+    RC(reset_indenter_force());
+    if (best_i < 0) RC(set_tile_shape(tx0, ty0, margin_));
+    else RC(set_tile_shape(cand[best_i][0], cand[best_i][1], margin_));
+    if (chosen) *chosen = best_i;
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+}  // namespace icempm

@@ -1,0 +1,264 @@
+// kernels_fp64.hip -- FP64 substep kernels for gfx950 (CDNA4).
+//
+// Replaces partition_kernel_p2g, partition_kernel_update_nodes and
+// partition_kernel_g2p (simulation/kernels.cuh:46-122, 164-301, 305-414).
+//
+// Layout: points are the reference's SoA (16 lines of `pitch` doubles); each
+// scene's grid is 3 lines (mass, momentum->velocity x, y) of `gpitch`
+// doubles with node (col, row) at col*gridY + row, col = gx + halo.
+//
+// The binned kernels work on tiles of TX x TY cells: a workgroup stages the
+// grid velocities of the tile's node window in LDS (G2P gathers), and sums
+// the P2G contributions of the tile's points in an LDS window (ds_add_f64)
+// that is flushed with one global atomic per touched node and line.  The
+// point runs of a tile come from the cell_start table of the last sort; a
+// point that drifted out of the window since then falls back to global memory.
+//
+// Built with -ffp-contract=off: every product/sum is rounded separately, as in
+// the FP64 oracle; the only source of difference is the order of the atomic sums.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "point_fp64.h"
+
+namespace icempm {
+namespace fp64 {
+
+// ---------------------------------------------------------------------------
+// Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
+{
+    constexpr bool DO_G2P = (MODE & kTileG2P) != 0;
+    constexpr bool DO_P2G = (MODE & kTileP2G) != 0;
+    extern __shared__ double lds[];
+    __shared__ long long run_start[kMaxTX];
+    __shared__ int run_pre[kMaxTX + 1];
+
+    const int WN = a.win_w * a.win_h;
+    double *vwx = lds, *vwy = lds + WN;
+    double *am = lds + (DO_G2P ? 2 * WN : 0), *apx = am + WN, *apy = apx + WN;
+    const int tid = threadIdx.x;
+    const int ntiles = *a.n_tiles;
+    const PtRef R{(double *)a.pts, a.pitch};
+
+    for (int w = xcd_block(); w < ntiles; w += gridDim.x) {
+        const TileItem ti = a.tiles[w];
+        const SceneDev &S = a.scenes[ti.scene];
+        const SceneConsts &c = S.c;
+        const int gridY = c.gridY;
+        const int wc0 = ti.col0 - 1 - a.margin, wr0 = ti.row0 - 1 - a.margin;
+        const double *gr = (const double *)S.grid[rbuf];
+        double *gw = (double *)S.grid[wbuf];
+        const long long gp = S.gpitch;
+        bool record = false;
+        if (DO_G2P)
+            record = record_mode >= 0 ? record_mode != 0 : (a.clocks[ti.scene].step % c.update_every) == 0;
+
+        // point runs of the tile's columns
+        if (tid < a.tx) {
+            const int col = ti.col0 + tid;
+            long long s0 = 0, s1 = 0;
+            if (col < c.ncols) {
+                const long long base = S.cell_off + (long long)col * gridY;
+                s0 = a.cell_start[base + ti.row0];
+                s1 = a.cell_start[base + min(ti.row0 + a.ty, gridY)];
+            }
+            run_start[tid] = s0;
+            run_pre[tid + 1] = (int)(s1 - s0);
+        }
+        // stage velocities / clear accumulators
+        for (int i = tid; i < WN; i += kBlock) {
+            if (DO_G2P) {
+                const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
+                double vx = 0, vy = 0;
+                if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
+                    const long long gi = (long long)col * gridY + row;
+                    vx = gr[gp + gi];
+                    vy = gr[2 * gp + gi];
+                }
+                vwx[i] = vx;
+                vwy[i] = vy;
+            }
+            if (DO_P2G) { am[i] = 0; apx[i] = 0; apy[i] = 0; }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            run_pre[0] = 0;
+            for (int k = 0; k < a.tx; k++) run_pre[k + 1] += run_pre[k];
+        }
+        __syncthreads();
+        const int total = run_pre[a.tx];
+
+        unsigned err = 0;
+        int cidx = 0;
+        for (int k0 = tid * kPPT; k0 < total; k0 += kBlock * kPPT) {
+#pragma unroll
+            for (int q = 0; q < kPPT; q++) {
+                const int k = k0 + q;
+                if (k >= total) break;
+                while (k >= run_pre[cidx + 1]) cidx++;
+                const long long pt = run_start[cidx] + (k - run_pre[cidx]);
+                long long ud = as_ll(R.at(kLineUtility, pt));
+                if (ud & kDisabled) continue;
+                double x, y, F[4], Jp, v[2], B[4];
+                load_state(R, pt, x, y, F, Jp);
+                if (DO_G2P) {
+                    double pp, qq;
+                    unsigned e;
+                    if (!g2p_point(c, gr, gp, vwx, vwy, wc0, wr0, a.win_w, a.win_h, x, y, v, B, F, Jp, ud, pp, qq,
+                                   e)) {
+                        err = max(err, e);
+                        continue;
+                    }
+                    store_state(R, pt, x, y, v, F, B, Jp, ud, record, pp, qq);
+                } else {
+                    v[0] = R.at(kLineVelX, pt);
+                    v[1] = R.at(kLineVelX + 1, pt);
+#pragma unroll
+                    for (int m = 0; m < 4; m++) B[m] = R.at(kLineBp00 + m, pt);
+                }
+                if (DO_P2G) {
+                    const unsigned e = p2g_point(c, x, y, v, B, F, Jp, ud,
+                        [&](int col, int row, double dm, double dpx, double dpy) {
+                            const int lx = col - wc0, ly = row - wr0;
+                            if (lx >= 0 && lx < a.win_w && ly >= 0 && ly < a.win_h) {
+                                const int li = lx * a.win_h + ly;
+                                atomicAdd(&am[li], dm);
+                                atomicAdd(&apx[li], dpx);
+                                atomicAdd(&apy[li], dpy);
+                            } else {
+                                const long long gi = (long long)col * gridY + row;
+                                atomicAdd(&gw[gi], dm);
+                                atomicAdd(&gw[gp + gi], dpx);
+                                atomicAdd(&gw[2 * gp + gi], dpy);
+                            }
+                        });
+                    err = max(err, e);
+                }
+            }
+        }
+        if (err) atomicMax(&a.err[ti.scene], err);
+        if (DO_P2G) {
+            __syncthreads();
+            for (int i = tid; i < WN; i += kBlock) {
+                const double m = am[i];
+                if (m == 0) continue;
+                const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
+                if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
+                const long long gi = (long long)col * gridY + row;
+                atomicAdd(&gw[gi], m);
+                atomicAdd(&gw[gp + gi], apx[i]);
+                atomicAdd(&gw[2 * gp + gi], apy[i]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grid update (kernels.cuh:164-301, live lines), one thread per node.  With
+// zero_other set it also clears the other accumulation grid for the next P2G.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
+{
+    for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
+        const NodeChunk nc = a.chunks[ch];
+        const SceneDev &S = a.scenes[nc.scene];
+        const SceneConsts &c = S.c;
+        const SceneClock clk = a.clocks[nc.scene];
+        double *g = (double *)S.grid[buf];
+        double *go = (double *)S.grid[buf ^ 1];
+        const long long gp = S.gpitch;
+        const double dt = c.dt;
+        for (int k = threadIdx.x; k < nc.count; k += kBlock) {
+            const long long idx = (long long)nc.node0 + k;
+            if (zero_other && go[idx] != 0) {
+                go[idx] = 0;
+                go[gp + idx] = 0;
+                go[2 * gp + idx] = 0;
+            }
+            const double mass = g[idx];
+            if (mass == 0) continue;
+            const double px = g[gp + idx], py = g[2 * gp + idx];
+            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+            const double gnx = (double)gix * c.h, gny = (double)giy * c.h;
+            double vx = px / mass, vy = py / mass;
+            vy -= c.dt_gravity;
+            const double sq = vx * vx + vy * vy;
+            if (sq > c.vmax_sq) {
+                const double r = sqrt(sq);
+                vx = vx / r * c.vmax;
+                vy = vy / r * c.vmax;
+            }
+            double n0 = gnx - clk.ind_x, n1 = gny - clk.ind_y;
+            if (n0 * n0 + n1 * n1 < c.ind_rsq) {
+                const double vr0 = vx - c.ind_vel, vr1 = vy;
+                const double z = n0 * n0 + n1 * n1;
+                if (z > 0) { const double r = sqrt(z); n0 = n0 / r; n1 = n1 / r; }
+                const double vn = vr0 * n0 + vr1 * n1;
+                if (vn < 0) {
+                    const double vt0 = vr0 - n0 * vn, vt1 = vr1 - n1 * vn;
+                    const double pv0 = vx, pv1 = vy;
+                    vx = c.ind_vel + vt0;
+                    vy = 0 + vt1;
+                    const double f0 = (pv0 - vx) * mass / dt, f1 = (pv1 - vy) * mass / dt;
+                    float angle = atan2f((float)n0, (float)n1);
+                    angle += kPi;
+                    angle *= c.n_ind_sub / (2 * kPi);
+                    const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
+                    atomicAdd(&S.ind_force[2 * index], f0);
+                    atomicAdd(&S.ind_force[2 * index + 1], f1);
+                }
+            }
+            if (giy <= 2) { vx = 0; vy = 0; }
+            if (giy >= c.gridY - 3 && vy > 0) vy = 0;
+            if (gix <= 2 && vx < 0) vx = 0;
+            else if (gix >= c.gridXTotal - 3 && vx > 0) vx = 0;
+            g[gp + idx] = vx;
+            g[2 * gp + idx] = vy;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+size_t tile_lds_bytes(const StepArgs &a, int mode)
+{
+    const size_t WN = (size_t)a.win_w * a.win_h;
+    size_t n = 0;
+    if (mode & kTileG2P) n += 2 * WN;
+    if (mode & kTileP2G) n += 3 * WN;
+    return n * sizeof(double);
+}
+
+hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
+                        hipStream_t s)
+{
+    if (a.tx > kMaxTX || n_blocks <= 0) return hipErrorInvalidValue;
+    const size_t lds = tile_lds_bytes(a, mode);
+    switch (mode) {
+    case kTileG2P:
+        hipLaunchKernelGGL(k_tiles<kTileG2P>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
+        break;
+    case kTileP2G:
+        hipLaunchKernelGGL(k_tiles<kTileP2G>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
+        break;
+    case kTileFused:
+        hipLaunchKernelGGL(k_tiles<kTileFused>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int n_blocks, hipStream_t s)
+{
+    if (n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_update, dim3(n_blocks), dim3(kBlock), 0, s, a, buf, zero_other);
+    return hipGetLastError();
+}
+
+}  // namespace fp64
+}  // namespace icempm

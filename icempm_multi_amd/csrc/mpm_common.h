@@ -1,0 +1,90 @@
+// mpm_common.h -- POD types shared by the host runtime and the HIP kernels.
+//
+// Every launch works over a table of scenes.  A partition of the reference
+// (GPU_Partition, gpu_partition.h:79-165) is a table of one scene; the batch
+// form is a table of many.  Per-scene constants are derived once from the
+// reference's SimParams (parameters_sim.cpp:128-165) by the host.
+#pragma once
+
+#include <stdint.h>
+
+namespace icempm {
+
+constexpr int kNumPtsLines = 16;     // HostSideSOA lines (parameters_sim.h:27-35)
+constexpr int kLineUtility = 0, kLineP = 1, kLineQ = 2, kLineJp = 3, kLinePosX = 4, kLineVelX = 6,
+              kLineFe00 = 8, kLineBp00 = 12;
+constexpr int64_t kCrushed = 0x10000, kDisabled = 0x20000, kLiquid = 0x40000;
+constexpr int kMaxIndenterSub = 1024;
+
+// Everything a kernel needs to know about one scene's physics and geometry.
+struct SceneConsts {
+    int gridX, gridY, halo, ncols;    // node column col = gx - xoff + halo, gx in [xoff-halo, xoff+gridX+halo)
+    int xoff, gridXTotal;             // GridX_offset of the partition, GridXTotal of the domain
+    int n_ind_sub, update_every;      // n_indenter_subdivisions, UpdateEveryNthStep
+    int frac_bits, origin, origin_y, pad_c;  // mixed form: x/h = xq*2^-frac_bits - origin (y: origin_y)
+    double h, hinv;
+    double dt, dt_vol_Dpinv, dt_gravity, vmax, vmax_sq;
+    double pmass, wmass;              // ParticleMass, WaterParticleMass
+    double mu, kappa;
+    double ics, its, iss, its2;       // Ice{Compressive,Tensile,Shear,Tensile2}Strength
+    double beta, grain_var, tan_phi, dp_threshold_p;
+    double ind_rsq, ind_vel, ind_x0, ind_y;
+};
+
+struct SceneDev {
+    SceneConsts c;
+    long long pt_off;       // first point of the scene in the SoA
+    int n_pts;              // points of the scene (after the last compaction)
+    int cap_pts;            // capacity of the scene's point range
+    void *grid[2];          // two accumulation grids (3 lines of gpitch elements, or 6 for mixed)
+    long long gpitch;       // elements per grid line (>= ncols*gridY)
+    long long cell_off;     // first entry of the scene in the cell_start array
+    int ncells;             // ncols*gridY
+    int n_tiles_x, n_tiles_y;
+    int pad0;
+    double *ind_force;      // 2*n_ind_sub doubles
+};
+
+// Device clock of a scene, advanced on the device once per substep so that a
+// captured graph can be replayed frame after frame (model.cpp:39-41,47).
+struct SceneClock {
+    double t;               // SimulationTime after the current substep
+    double ind_x;           // indenter_x_initial + t*IndVelocity
+    double ind_y;           // indenter_y
+    int step;               // SimulationStep + substeps done before the current one (recordPQ test)
+    int next_step;          // step of the next substep
+};
+
+// A tile of TX x TY cells of one scene: the unit of the binned P2G / fused G2P.
+struct TileItem {
+    int scene;
+    int col0, row0;
+    int npts;
+};
+
+// A contiguous run of grid nodes of one scene: the unit of the grid update.
+struct NodeChunk {
+    int scene;
+    int node0;
+    int count;
+    int pad;
+};
+
+// Arguments common to all stepping kernels.
+struct StepArgs {
+    const SceneDev *scenes;
+    SceneClock *clocks;
+    void *pts;                // SoA base: kNumPtsLines lines of `pitch` elements
+    long long pitch;
+    const int *cell_start;    // per scene ncells+1 global point indices (from the last sort)
+    const TileItem *tiles;
+    const int *n_tiles;       // device-side count of live tiles
+    const NodeChunk *chunks;
+    int n_chunks;
+    int nscenes;
+    unsigned *err;            // per-scene error byte (max over points)
+    int tx, ty, margin;       // tile shape and drift margin of the LDS windows
+    int win_w, win_h;         // tx+2+2*margin, ty+2+2*margin
+};
+
+}  // namespace icempm

@@ -1,0 +1,161 @@
+// point_fp64.h -- FP64 per-point G2P and P2G steps shared by the binned and
+// unbinned kernels (simulation/kernels.cuh:46-122, 305-414).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "physics_fp64.h"
+
+namespace icempm {
+namespace fp64 {
+
+constexpr int kBlock = 256;
+constexpr int kPPT = 4;          // consecutive points per thread and round
+constexpr int kMaxTX = 32;
+
+__device__ __forceinline__ long long as_ll(double d) { return __double_as_longlong(d); }
+__device__ __forceinline__ double as_d(long long v) { return __longlong_as_double(v); }
+
+struct PtRef {
+    double *P;
+    long long pitch;
+    __device__ __forceinline__ double &at(int line, long long i) const { return P[line * pitch + i]; }
+};
+
+// Logical block index that keeps consecutive work items on one XCD: workgroups
+// are dispatched round-robin over the 8 XCDs, so block b runs on XCD b%8.
+__device__ __forceinline__ int xcd_block()
+{
+    const int G = gridDim.x;
+    if ((G & 7) != 0) return blockIdx.x;
+    return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+}
+
+// G2P of one point: gathers v and C (kernels.cuh:344-372), advects x, updates F
+// (375-376) and runs the constitutive branch (378-391).  Returns false (and
+// sets the error byte) if the stencil leaves the grid.
+__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const double *g, long long gpitch,
+                                          const double *vwx, const double *vwy, int wc0, int wr0, int win_w,
+                                          int win_h, double &x, double &y, double v[2], double B[4], double F[4],
+                                          double &Jp, long long &ud, double &p_out, double &q_out, unsigned &e)
+{
+    int bx, by;
+    double fx, fy;
+    base_fx(c, x, bx, fx);
+    base_fx(c, y, by, fy);
+    const int bxc = bx - c.xoff + c.halo;
+    e = stencil_error(c, bxc, by);
+    if (e) return false;
+    double wx[3], wy[3];
+    weights(fx, wx);
+    weights(fy, wy);
+    const int lx = bxc - wc0, ly = by - wr0;
+    const bool inwin = vwx && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
+    v[0] = v[1] = 0;
+    B[0] = B[1] = B[2] = B[3] = 0;
+    const double s4 = 4. * c.hinv;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double nvx, nvy;
+            if (inwin) {
+                const int li = (lx + i) * win_h + ly + j;
+                nvx = vwx[li];
+                nvy = vwy[li];
+            } else {
+                const long long gi = (long long)(bxc + i) * c.gridY + by + j;
+                nvx = g[gpitch + gi];
+                nvy = g[2 * gpitch + gi];
+            }
+            const double d0 = i - fx, d1 = j - fy;
+            const double w = wx[i] * wy[j];
+            v[0] += w * nvx;
+            v[1] += w * nvy;
+            const double s = s4 * w;
+            B[0] += s * (nvx * d0);
+            B[1] += s * (nvx * d1);
+            B[2] += s * (nvy * d0);
+            B[3] += s * (nvy * d1);
+        }
+    x += v[0] * c.dt;
+    y += v[1] * c.dt;
+    const double A[4] = {1.0 + c.dt * B[0], c.dt * B[1], c.dt * B[2], 1.0 + c.dt * B[3]};
+    mul2(A, F, F);
+    constitutive(c, F, Jp, ud, A, p_out, q_out);
+    return true;
+}
+
+// P2G contributions of one point (kernels.cuh:59-121).  Calls sink(col, row,
+// dm, dpx, dpy) for the 9 stencil nodes; returns the error byte (0 = ok).
+template <class Sink>
+__device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, double x, double y, const double v[2],
+                                              const double B[4], const double F[4], double Jp, long long ud,
+                                              Sink &&sink)
+{
+    const bool liquid = (ud & kLiquid) != 0;
+    const double mass = liquid ? c.wmass : c.pmass;
+    double tau[4];
+    if (liquid) water(c, Jp, tau);
+    else kirchhoff(c, F, tau);
+    double S[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) S[k] = mass * B[k] - c.dt_vol_Dpinv * tau[k];
+    int bx, by;
+    double fx, fy;
+    base_fx(c, x, bx, fx);
+    base_fx(c, y, by, fy);
+    const int bxc = bx - c.xoff + c.halo;
+    const unsigned e = stencil_error(c, bxc, by);
+    if (e) return e;
+    double wx[3], wy[3];
+    weights(fx, wx);
+    weights(fy, wy);
+    const double mv0 = v[0] * mass, mv1 = v[1] * mass;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double W = wx[i] * wy[j];
+            const double d0 = (i - fx) * c.h, d1 = (j - fy) * c.h;
+            const double t0 = mv0 + (S[0] * d0 + S[1] * d1);
+            const double t1 = mv1 + (S[2] * d0 + S[3] * d1);
+            sink(bxc + i, by + j, W * mass, W * t0, W * t1);
+        }
+    return 0;
+}
+
+__device__ __forceinline__ void load_state(const PtRef &R, long long pt, double &x, double &y, double F[4],
+                                           double &Jp)
+{
+    x = R.at(kLinePosX, pt);
+    y = R.at(kLinePosX + 1, pt);
+#pragma unroll
+    for (int k = 0; k < 4; k++) F[k] = R.at(kLineFe00 + k, pt);
+    Jp = R.at(kLineJp, pt);
+}
+
+__device__ __forceinline__ void store_state(const PtRef &R, long long pt, double x, double y, const double v[2],
+                                            const double F[4], const double B[4], double Jp, long long ud,
+                                            bool record, double p, double q)
+{
+    R.at(kLinePosX, pt) = x;
+    R.at(kLinePosX + 1, pt) = y;
+    R.at(kLineVelX, pt) = v[0];
+    R.at(kLineVelX + 1, pt) = v[1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        R.at(kLineFe00 + k, pt) = F[k];
+        R.at(kLineBp00 + k, pt) = B[k];
+    }
+    R.at(kLineJp, pt) = Jp;
+    R.at(kLineUtility, pt) = as_d(ud);
+    if (record) {
+        R.at(kLineP, pt) = p;
+        R.at(kLineQ, pt) = q;
+    }
+}
+
+}  // namespace fp64
+}  // namespace icempm

@@ -1,0 +1,268 @@
+/*
+ * icempm.h -- C-ABI of the MI355X-native 2D MPM ice stepper.
+ *
+ * The reference (igorg520b/iceMPM_multi) has no FFI: its drop-in boundary is the
+ * C++ class surface of GPU_Partition (simulation/gpu_partition.h:79-165) and
+ * GPU_Implementation5 (simulation/gpu_implementation5.h:25-59), driven by
+ * icy::Model::Step (simulation/model.cpp:27-113).  Every entry point below names
+ * the reference method it replaces.  Plain pointers and sizes only; every call
+ * returns an int status (0 = ok, <0 = ICEMPM_ERR_*); device-side stencil faults
+ * are reported through the error byte (70..73, 5) exactly as the reference's
+ * gpu_error_indicator (simulation/kernels.cuh:20,94-97,506).
+ *
+ * Threading: one handle is used from one host thread at a time.  All work is
+ * asynchronous on the handle's own HIP stream; the calls that return host data
+ * (transfer_from_device, get_*) synchronise that stream first.
+ */
+#ifndef ICEMPM_H
+#define ICEMPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Parameters: field-for-field mirror of icy::SimParams                       */
+/* (simulation/parameters_sim.h:37-85).  The reference copies that struct by  */
+/* sizeof into __constant__ memory (gpu_partition.cu:373), so the member       */
+/* order and types are kept identical: the bytes of a reference SimParams can  */
+/* be passed to icempm_partition_update_constants unchanged.                   */
+/* ------------------------------------------------------------------------- */
+typedef struct icempm_simparams {
+    int n_indenter_subdivisions;
+    int tpb_P2G, tpb_Upd, tpb_G2P;      /* launch hints; accepted, autotuner decides */
+
+    int nPtsTotal;
+    int GridXTotal, GridY, GridTotal;
+    double GridXDimension;
+
+    double InitialTimeStep, SimulationEndTime;
+    int UpdateEveryNthStep;
+    int SimulationStep;
+    double SimulationTime;
+
+    double Gravity, Density, WaterDensity, PoissonsRatio, YoungsModulus;
+    double lambda, mu, kappa;
+
+    double IceCompressiveStrength, IceTensileStrength, IceShearStrength, IceTensileStrength2;
+    double NACC_beta, NACC_M, NACC_Msq;
+
+    double DP_tan_phi, DP_threshold_p;
+
+    double IndDiameter, IndRSq, IndVelocity, IndDepth;
+
+    double cellsize, cellsize_inv, Dp_inv;
+    double xmin, xmax, ymin, ymax;
+    int nxmin, nxmax, nymin, nymax;
+
+    double ParticleVolume, ParticleMass, WaterParticleMass, ParticleViewSize;
+
+    double indenter_x, indenter_x_initial, indenter_y, indenter_y_initial;
+    double Volume;
+    int SetupType;
+    double GrainVariability;
+
+    int GridHaloSize;
+    double ExtraSpaceForIncomingPoints;
+    double PointsTransferBufferFraction;
+    double RebalanceThresholdFreeSpaceRemaining;
+    double RebalanceThresholdDisabledPercentage;
+    int PointTransferFrequency;
+
+    int nPartitions;
+    int VectorCapacity_transfer;
+
+    double dt_vol_Dpinv, dt_Gravity, vmax, vmax_squared;
+    int gbOffset;
+} icempm_simparams;
+
+/* Host particle SoA line indices (parameters_sim.h:27-35): a HostSideSOA buffer
+ * holds ICEMPM_N_PTS_ARRAYS lines of `capacity` doubles each. */
+#define ICEMPM_IDX_UTILITY_DATA 0
+#define ICEMPM_IDX_P            1
+#define ICEMPM_IDX_Q            2
+#define ICEMPM_IDX_JP_INV       3
+#define ICEMPM_IDX_POSX         4
+#define ICEMPM_IDX_VELX         6
+#define ICEMPM_IDX_FE00         8
+#define ICEMPM_IDX_BP00         12
+#define ICEMPM_N_PTS_ARRAYS     16
+#define ICEMPM_N_GRID_ARRAYS    3
+
+/* utility_data bits (kernels.cuh:12-14, proxypoint2d.cpp:90-176) */
+#define ICEMPM_STATUS_CRUSHED   0x10000LL
+#define ICEMPM_STATUS_DISABLED  0x20000LL
+#define ICEMPM_STATUS_LIQUID    0x40000LL
+
+/* Precision forms */
+#define ICEMPM_PRECISION_FP64   0   /* FP64 state and arithmetic (the reference's form) */
+#define ICEMPM_PRECISION_MIXED  1   /* FP32 state, compensated grid sums, FP64 islands */
+
+/* Status codes */
+#define ICEMPM_OK               0
+#define ICEMPM_ERR_ARGUMENT    (-1)
+#define ICEMPM_ERR_HIP         (-2)
+#define ICEMPM_ERR_CAPACITY    (-3)
+#define ICEMPM_ERR_STATE       (-4)
+#define ICEMPM_ERR_NOMEM       (-5)
+#define ICEMPM_ERR_DEVICE      (-6)   /* device error byte set; see get_error_code */
+
+/* Device error byte values (kernels.cuh:94-97, 506) */
+#define ICEMPM_DEVERR_LEFT      70
+#define ICEMPM_DEVERR_BOTTOM    71
+#define ICEMPM_DEVERR_RIGHT     72
+#define ICEMPM_DEVERR_TOP       73
+#define ICEMPM_DEVERR_NO_SPACE  5
+
+/* Flags for the frame-level step calls */
+#define ICEMPM_STEP_GRAPH       0x1   /* capture the substeps into a hipGraph and replay it */
+#define ICEMPM_STEP_RESORT      0x2   /* re-bin and re-sort points on device before the substeps */
+#define ICEMPM_STEP_TIMINGS     0x4   /* record per-phase hipEvents (timing_* fields) */
+#define ICEMPM_STEP_DIRECT_P2G  0x8   /* force the unbinned one-thread-per-point P2G */
+#define ICEMPM_STEP_UNFUSED     0x10  /* keep G2P and the next P2G as separate launches */
+
+/* Library identification and the last error text of the calling thread. */
+const char *icempm_version(void);
+const char *icempm_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* Parameters and point preparation (host only, no device needed)             */
+/* ------------------------------------------------------------------------- */
+/* icy::SimParams::Reset (parameters_sim.cpp:5-60). */
+int icempm_params_reset(icempm_simparams *prms);
+/* icy::SimParams::ParseFile (parameters_sim.cpp:63-126) on the file's text; the
+ * InputRawPoints value is copied to input_raw_points (n bytes, may be null). */
+int icempm_params_parse_json(icempm_simparams *prms, const char *json_text, char *input_raw_points, size_t n);
+/* ComputeCamClayParams2 + ComputeHelperVariables (parameters_sim.cpp:128-165). */
+int icempm_params_derive(icempm_simparams *prms);
+/* icy::SnapshotManager::LoadRawPoints after the HDF5 read (snapshotmanager.cpp:57-110):
+ * x, y (lines 4-5) and llGrainIDs (line 0) of n_points points are in hssoa;
+ * offsets the block, removes disabled points and sorts by cell key, sets
+ * Fe = I and Jp_inv = 1, places the indenter and sets the particle volume from
+ * the float32 `volume` attribute.  *n_kept = points left. */
+int icempm_prepare_raw_points(icempm_simparams *prms, double *hssoa, unsigned hssoa_capacity, int n_points,
+                              float volume, int *n_kept);
+
+/* ------------------------------------------------------------------------- */
+/* Partition: one grid + point set on one device (GPU_Partition)              */
+/* ------------------------------------------------------------------------- */
+typedef struct icempm_partition icempm_partition;
+
+/* GPU_Partition::initialize (gpu_partition.cu:113-134): stream, events. */
+int icempm_partition_create(int device, int partition_id, int precision, icempm_partition **out);
+/* GPU_Partition::~GPU_Partition (gpu_partition.cu:88-111). */
+int icempm_partition_destroy(icempm_partition *p);
+/* GPU_Partition::allocate (gpu_partition.cu:137-197): points and grid capacity. */
+int icempm_partition_allocate(icempm_partition *p, int n_points_capacity, int grid_x_capacity);
+/* GPU_Partition::update_constants (gpu_partition.cu:200-208): SimParams bytes. */
+int icempm_partition_update_constants(icempm_partition *p, const icempm_simparams *prms, size_t prms_bytes);
+/* GPU_Partition::GridX_offset / GridX_partition fields (gpu_partition.h:124-125),
+ * set by GPU_Implementation5::split_hssoa_into_partitions (gpu_implementation5.cpp:284-326). */
+int icempm_partition_set_grid_span(icempm_partition *p, int gridx_offset, int gridx_partition);
+/* GPU_Partition::transfer_points_from_soa_to_device (gpu_partition.cu:49-68):
+ * n_points points starting at point_idx_offset of a HostSideSOA buffer whose
+ * lines are hssoa_capacity doubles apart. */
+int icempm_partition_transfer_points_from_soa_to_device(icempm_partition *p, const double *hssoa,
+                                                        unsigned hssoa_capacity, int point_idx_offset,
+                                                        int n_points);
+/* GPU_Partition::transfer_from_device (gpu_partition.cu:16-46): points, indenter
+ * force accumulator and error byte; synchronises the stream. */
+int icempm_partition_transfer_from_device(icempm_partition *p, double *hssoa, unsigned hssoa_capacity,
+                                          int point_idx_offset);
+/* GPU_Partition::reset_grid (gpu_partition.cu:221-228). */
+int icempm_partition_reset_grid(icempm_partition *p);
+/* GPU_Partition::reset_indenter_force_accumulator (gpu_partition.cu:212-218). */
+int icempm_partition_reset_indenter_force_accumulator(icempm_partition *p);
+/* GPU_Partition::p2g (gpu_partition.cu:230-242) -> partition_kernel_p2g (kernels.cuh:46-122). */
+int icempm_partition_p2g(icempm_partition *p);
+/* GPU_Partition::update_nodes (gpu_partition.cu:244-257) -> kernels.cuh:164-301.
+ * The indenter centre is an argument, as the reference reads prms->indenter_x/y
+ * on the host at launch time (gpu_partition.cu:251). */
+int icempm_partition_update_nodes(icempm_partition *p, double simulation_time, double indenter_x,
+                                  double indenter_y);
+/* GPU_Partition::g2p (gpu_partition.cu:259-293) -> kernels.cuh:305-414.
+ * Point migration (kernels.cuh:416-461) is a multi-GPU feature and not built. */
+int icempm_partition_g2p(icempm_partition *p, int recordPQ);
+/* GPU_Implementation5::synchronize (gpu_implementation5.cpp:439-446). */
+int icempm_partition_synchronize(icempm_partition *p);
+/* Host copy of the indenter force accumulator (n_indenter_subdivisions x 2
+ * doubles), as read in GPU_Implementation5::transfer_from_device (gpu_implementation5.cpp:392-396). */
+int icempm_partition_get_indenter_force(icempm_partition *p, double *out, int n_doubles);
+/* The error byte (gpu_error_indicator), read back after a sync; 0 = none. */
+int icempm_partition_get_error_code(icempm_partition *p, uint8_t *code);
+/* nPts_partition and the number of disabled points (gpu_partition.h:122-123). */
+int icempm_partition_get_point_count(icempm_partition *p, int *n_points, int *n_disabled);
+/* GPU_Partition timing_* fields (gpu_partition.h:156-162) in ms summed since the
+ * last reset_indenter_force_accumulator: out[0]=P2G, [1]=halo (0), [2]=update,
+ * [3]=G2P, [4]=sent (0), [5]=accepted (0), [6]=total, [7]=sort. */
+int icempm_partition_get_timings(icempm_partition *p, float *out, int n);
+
+/* Device time per kernel class since the last reset_indenter_force_accumulator,
+ * recorded with hipEvents around each launch when ICEMPM_STEP_TIMINGS is set:
+ * class 0 clock, 1 zero-fill, 2 P2G, 3 fused G2P+P2G, 4 G2P, 5 grid update, 6 sort.
+ * ms[k] = summed milliseconds, count[k] = launches. */
+int icempm_partition_get_kernel_stats(icempm_partition *p, double *ms, long long *count, int n);
+
+/* Frame-level substep loop of icy::Model::Step (model.cpp:38-54) run on the
+ * device: n_substeps x (reset_grid, p2g, update_nodes(t), g2p(recordPQ)), with
+ * t advanced by repeated addition of InitialTimeStep from prms.SimulationTime,
+ * indenter_x = indenter_x_initial + t*IndVelocity, and recordPQ on substeps
+ * where (SimulationStep + k) % UpdateEveryNthStep == 0.  SimulationTime and
+ * SimulationStep of the partition's parameters advance by the substeps run. */
+int icempm_partition_step(icempm_partition *p, int n_substeps, int flags);
+/* Current SimulationTime / SimulationStep of the partition. */
+int icempm_partition_get_clock(icempm_partition *p, double *simulation_time, int *simulation_step);
+
+/* HostSideSOA::RemoveDisabledAndSort (host_side_soa.cpp:50-61) run on the
+ * device: disabled points are removed and the rest ordered by the cell key
+ * (int)(x/h+0.5)*GridY + (int)(y/h+0.5) (proxypoint2d.cpp:164-169), ties kept
+ * in their previous order (a stable sort). */
+int icempm_partition_sort_points(icempm_partition *p);
+/* Binning results of the last sort, for checks: keys[n] per sorted point,
+ * perm[n] = previous index of each sorted point, and the per-cell counts over
+ * the grid's cell range (n_cells entries, cell = key + GridHaloSize*GridY). */
+int icempm_partition_get_bins(icempm_partition *p, int32_t *keys, int32_t *perm, int n,
+                              int32_t *cell_counts, int n_cells);
+
+/* Copy of grid buffer `buf` (0 or 1) as 3 lines of n_nodes doubles (mass,
+ * momentum or velocity x, y), n_nodes = GridY*(GridXTotal + 2*GridHaloSize),
+ * node (gx, gy) at (gx+GridHaloSize)*GridY + gy.  For inspection and tests; the
+ * mixed form returns hi+lo of its compensated pairs.  No reference counterpart
+ * (the reference keeps the grid internal). */
+int icempm_partition_get_grid(icempm_partition *p, int buf, double *out, long long n_nodes);
+
+/* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
+ * the current point set and keeps the fastest.  Results (ms per launch) of each
+ * candidate go to out_ms if non-null. */
+int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen);
+
+/* ------------------------------------------------------------------------- */
+/* Batch: many independent scenes stepped by one persistent launch per phase   */
+/* (no reference counterpart; the reference runs one scene per process).       */
+/* ------------------------------------------------------------------------- */
+typedef struct icempm_batch icempm_batch;
+
+int icempm_batch_create(int device, int precision, icempm_batch **out);
+int icempm_batch_destroy(icempm_batch *b);
+/* Adds a scene: its parameters and n_points points from a HostSideSOA buffer. */
+int icempm_batch_add_scene(icempm_batch *b, const icempm_simparams *prms, size_t prms_bytes,
+                           const double *hssoa, unsigned hssoa_capacity, int n_points, int *scene_id);
+/* Allocates device memory for all scenes, uploads them and bins them. */
+int icempm_batch_commit(icempm_batch *b);
+/* n_substeps of every scene (each with its own clock), ICEMPM_STEP_* flags. */
+int icempm_batch_step(icempm_batch *b, int n_substeps, int flags);
+int icempm_batch_transfer_from_device(icempm_batch *b, int scene_id, double *hssoa, unsigned hssoa_capacity);
+int icempm_batch_get_indenter_force(icempm_batch *b, int scene_id, double *out, int n_doubles);
+int icempm_batch_get_error_code(icempm_batch *b, uint8_t *code);
+int icempm_batch_get_scene_count(icempm_batch *b, int *n_scenes, long long *n_points_total);
+/* Average device time per substep (ms) of the last icempm_batch_step call. */
+int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICEMPM_H */
