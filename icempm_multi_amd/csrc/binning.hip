@@ -166,8 +166,8 @@ __global__ void __launch_bounds__(kBlock) k_gather(StepArgs a, BinWork w, const 
 }
 
 // Points per tile (TX x TY cells) from cell_start; one thread per tile.
-__global__ void __launch_bounds__(kBlock) k_tile_count(StepArgs a, int scene_base_tile_unused, const int *tile_off,
-                                                       int n_tiles_all, int *tile_n)
+__global__ void __launch_bounds__(kBlock) k_tile_count(StepArgs a, const int *tile_off, int n_tiles_all, int *tile_n,
+                                                       int *tile_flag)
 {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n_tiles_all) return;
@@ -189,6 +189,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_count(StepArgs a, int scene_bas
         n += a.cell_start[base + min(row0 + a.ty, S.c.gridY)] - a.cell_start[base + row0];
     }
     tile_n[t] = n;
+    tile_flag[t] = n > 0 ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kBlock) k_tile_emit(StepArgs a, const int *tile_off, int n_tiles_all,
@@ -265,12 +266,14 @@ hipError_t launch_build_tiles(const StepArgs &a, const SceneDev *host_scenes, in
                               TileItem *tiles, int *n_tiles, long long max_tiles, hipStream_t s)
 {
     (void)w;
-    // tile_off, tile_n and tile_pos live in the tail of `tiles` storage: the
-    // caller sized it as max_tiles TileItems + 3*(max_tiles+nscenes+1) ints.
+    // tile_off, tile_n, tile_flag and tile_pos live in the tail of `tiles`
+    // storage: the caller sized it as max_tiles TileItems followed by
+    // 4*(max_tiles+nscenes+1) + max_tiles/1000 + 64 ints.
     int *ibase = (int *)(tiles + max_tiles);
     int *tile_off = ibase;
     int *tile_n = tile_off + nscenes + 1;
-    int *tile_pos = tile_n + max_tiles;
+    int *tile_flag = tile_n + max_tiles;
+    int *tile_pos = tile_flag + max_tiles;
     int *scan_tmp = tile_pos + max_tiles;
     long long total = 0;
     std::vector<int> off(nscenes + 1);
@@ -283,8 +286,9 @@ hipError_t launch_build_tiles(const StepArgs &a, const SceneDev *host_scenes, in
     hipError_t e = hipMemcpyAsync(tile_off, off.data(), sizeof(int) * (nscenes + 1), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     const unsigned g = nblk(total, kBlock);
-    hipLaunchKernelGGL(k_tile_count, dim3(g), dim3(kBlock), 0, s, a, 0, tile_off, (int)total, tile_n);
-    if ((e = scan_exclusive(tile_n, tile_pos, total, scan_tmp, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tile_count, dim3(g), dim3(kBlock), 0, s, a, tile_off, (int)total, tile_n, tile_flag);
+    // positions of the non-empty tiles: exclusive scan of the 0/1 flags
+    if ((e = scan_exclusive(tile_flag, tile_pos, total, scan_tmp, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_tile_emit, dim3(g), dim3(kBlock), 0, s, a, tile_off, (int)total, tile_n, tile_pos, tiles,
                        n_tiles);
     e = hipGetLastError();

@@ -128,6 +128,7 @@ private:
     TileItem *d_tiles_ = nullptr;
     int *d_n_tiles_ = nullptr;
     long long max_tiles_ = 0;
+    int n_live_tiles_ = 0;
     int tx_ = 4, ty_ = 32, margin_ = 1;
     std::vector<SceneDev> host_dev_;
 

@@ -382,7 +382,7 @@ int Engine::alloc_bins()
         mt += ((ncols + tx_ - 1) / tx_) * ((s.prms.GridY + ty_ - 1) / ty_);
     }
     max_tiles_ = mt;
-    const size_t tbytes = sizeof(TileItem) * mt + sizeof(int) * (3 * (mt + nscenes() + 1) + mt / 1000 + 64) + 64;
+    const size_t tbytes = sizeof(TileItem) * mt + sizeof(int) * (4 * (mt + nscenes() + 1) + mt / 1000 + 64) + 64;
     HIPCHK(hipMalloc(&tile_mem_, tbytes));
     HIPCHK(hipMemsetAsync(tile_mem_, 0, tbytes, stream_));
     d_tiles_ = (TileItem *)tile_mem_;
@@ -438,7 +438,7 @@ int Engine::set_tile_shape(int tx, int ty, int margin)
         mt += ((ncols + tx_ - 1) / tx_) * ((s.prms.GridY + ty_ - 1) / ty_);
     }
     max_tiles_ = mt;
-    const size_t tbytes = sizeof(TileItem) * mt + sizeof(int) * (3 * (mt + nscenes() + 1) + mt / 1000 + 64) + 64;
+    const size_t tbytes = sizeof(TileItem) * mt + sizeof(int) * (4 * (mt + nscenes() + 1) + mt / 1000 + 64) + 64;
     HIPCHK(hipMalloc(&tile_mem_, tbytes));
     HIPCHK(hipMemsetAsync(tile_mem_, 0, tbytes, stream_));
     d_tiles_ = (TileItem *)tile_mem_;
@@ -753,6 +753,13 @@ int Engine::build_tiles()
 {
     const StepArgs a = args();
     HIPCHK(launch_build_tiles(a, host_dev_.data(), nscenes(), bins_, d_tiles_, d_n_tiles_, max_tiles_, stream_));
+    int nt = 0;
+    HIPCHK(hipMemcpy(&nt, d_n_tiles_, sizeof(int), hipMemcpyDeviceToHost));
+    if (nt < 0 || nt > max_tiles_) {
+        bins_valid_ = false;
+        return fail(ICEMPM_ERR_STATE, "build_tiles: tile count out of range");
+    }
+    n_live_tiles_ = nt;
     return ICEMPM_OK;
 }
 
