@@ -207,14 +207,14 @@ int Engine::build_scene_dev(int i, SceneDev &d) const
     d.pt_off = s.pt_off;
     d.n_pts = s.n_pts;
     d.cap_pts = s.cap_pts;
-    d.grid[0] = s.grid[0];
-    d.grid[1] = s.grid[1];
+    d.grid_off[0] = (char *)s.grid[0] - (char *)grid_mem_;
+    d.grid_off[1] = (char *)s.grid[1] - (char *)grid_mem_;
     d.gpitch = s.grid_elems;
     d.cell_off = s.cells_off;
     d.ncells = c.ncols * c.gridY;
     d.n_tiles_x = (c.ncols + tx_ - 1) / tx_;
     d.n_tiles_y = (c.gridY + ty_ - 1) / ty_;
-    d.ind_force = s.ind_force;
+    d.force_off = s.ind_force - ind_force_mem_;
     return ICEMPM_OK;
 }
 
@@ -400,6 +400,8 @@ StepArgs Engine::args() const
 {
     StepArgs a{};
     a.scenes = d_scenes_;
+    a.grid_mem = (char *)grid_mem_;
+    a.force_mem = ind_force_mem_;
     a.clocks = d_clocks_;
     a.pts = pts_[0];
     a.pitch = pitch_;

@@ -26,19 +26,22 @@ namespace fp64 {
 
 // ---------------------------------------------------------------------------
 // Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
+// Register budget capped at 3 waves per SIMD (<= 168 VGPRs, no spills).
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_tiles(StepArgs a, int rbuf, int wbuf,
+                                                                                        int record_mode)
 {
     constexpr bool DO_G2P = (MODE & kTileG2P) != 0;
     constexpr bool DO_P2G = (MODE & kTileP2G) != 0;
-    extern __shared__ double lds[];
+    extern __shared__ double lds_raw[];
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
 
     const int WN = a.win_w * a.win_h;
-    double *vwx = lds, *vwy = lds + WN;
-    double *am = lds + (DO_G2P ? 2 * WN : 0), *apx = am + WN, *apy = apx + WN;
+    lds_d *lds = (lds_d *)lds_raw;
+    lds_d *vwx = lds, *vwy = lds + WN;
+    lds_d *am = lds + (DO_G2P ? 2 * WN : 0), *apx = am + WN, *apy = apx + WN;
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(double *)a.pts, a.pitch};
@@ -49,8 +52,8 @@ __global__ void __launch_bounds__(kBlock) k_tiles(StepArgs a, int rbuf, int wbuf
         const SceneConsts &c = S.c;
         const int gridY = c.gridY;
         const int wc0 = ti.col0 - 1 - a.margin, wr0 = ti.row0 - 1 - a.margin;
-        const double *gr = (const double *)S.grid[rbuf];
-        double *gw = (double *)S.grid[wbuf];
+        const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rbuf]);
+        glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
         bool record = false;
         if (DO_G2P)
@@ -91,52 +94,49 @@ __global__ void __launch_bounds__(kBlock) k_tiles(StepArgs a, int rbuf, int wbuf
         __syncthreads();
         const int total = run_pre[a.tx];
 
+        // points of the tile, strided over the workgroup: consecutive lanes take
+        // consecutive points of a run, so every SoA access is coalesced
         unsigned err = 0;
         int cidx = 0;
-        for (int k0 = tid * kPPT; k0 < total; k0 += kBlock * kPPT) {
-#pragma unroll
-            for (int q = 0; q < kPPT; q++) {
-                const int k = k0 + q;
-                if (k >= total) break;
-                while (k >= run_pre[cidx + 1]) cidx++;
-                const long long pt = run_start[cidx] + (k - run_pre[cidx]);
-                long long ud = as_ll(R.at(kLineUtility, pt));
-                if (ud & kDisabled) continue;
-                double x, y, F[4], Jp, v[2], B[4];
-                load_state(R, pt, x, y, F, Jp);
-                if (DO_G2P) {
-                    double pp, qq;
-                    unsigned e;
-                    if (!g2p_point(c, gr, gp, vwx, vwy, wc0, wr0, a.win_w, a.win_h, x, y, v, B, F, Jp, ud, pp, qq,
-                                   e)) {
-                        err = max(err, e);
-                        continue;
-                    }
-                    store_state(R, pt, x, y, v, F, B, Jp, ud, record, pp, qq);
-                } else {
-                    v[0] = R.at(kLineVelX, pt);
-                    v[1] = R.at(kLineVelX + 1, pt);
-#pragma unroll
-                    for (int m = 0; m < 4; m++) B[m] = R.at(kLineBp00 + m, pt);
-                }
-                if (DO_P2G) {
-                    const unsigned e = p2g_point(c, x, y, v, B, F, Jp, ud,
-                        [&](int col, int row, double dm, double dpx, double dpy) {
-                            const int lx = col - wc0, ly = row - wr0;
-                            if (lx >= 0 && lx < a.win_w && ly >= 0 && ly < a.win_h) {
-                                const int li = lx * a.win_h + ly;
-                                atomicAdd(&am[li], dm);
-                                atomicAdd(&apx[li], dpx);
-                                atomicAdd(&apy[li], dpy);
-                            } else {
-                                const long long gi = (long long)col * gridY + row;
-                                atomicAdd(&gw[gi], dm);
-                                atomicAdd(&gw[gp + gi], dpx);
-                                atomicAdd(&gw[2 * gp + gi], dpy);
-                            }
-                        });
+#pragma unroll 1
+        for (int k = tid; k < total; k += kBlock) {
+            while (k >= run_pre[cidx + 1]) cidx++;
+            const long long pt = run_start[cidx] + (k - run_pre[cidx]);
+            long long ud = as_ll(R.at(kLineUtility, pt));
+            if (ud & kDisabled) continue;
+            double x, y, F[4], Jp, v[2], B[4];
+            load_state(R, pt, x, y, F, Jp);
+            if (DO_G2P) {
+                double pp, qq;
+                unsigned e;
+                if (!g2p_point(c, gr, gp, vwx, vwy, wc0, wr0, a.win_w, a.win_h, x, y, v, B, F, Jp, ud, pp, qq, e)) {
                     err = max(err, e);
+                    continue;
                 }
+                store_state(R, pt, x, y, v, F, B, Jp, ud, record, pp, qq);
+            } else {
+                v[0] = R.at(kLineVelX, pt);
+                v[1] = R.at(kLineVelX + 1, pt);
+#pragma unroll
+                for (int m = 0; m < 4; m++) B[m] = R.at(kLineBp00 + m, pt);
+            }
+            if (DO_P2G) {
+                const unsigned e = p2g_point(c, x, y, v, B, F, Jp, ud,
+                    [&](int col, int row, double dm, double dpx, double dpy) {
+                        const int lx = col - wc0, ly = row - wr0;
+                        if (lx >= 0 && lx < a.win_w && ly >= 0 && ly < a.win_h) {
+                            const int li = lx * a.win_h + ly;
+                            lds_add(am + li, dm);
+                            lds_add(apx + li, dpx);
+                            lds_add(apy + li, dpy);
+                        } else {
+                            const long long gi = (long long)col * gridY + row;
+                            glb_add(gw + gi, dm);
+                            glb_add(gw + gp + gi, dpx);
+                            glb_add(gw + 2 * gp + gi, dpy);
+                        }
+                    });
+                err = max(err, e);
             }
         }
         if (err) atomicMax(&a.err[ti.scene], err);
@@ -148,9 +148,9 @@ __global__ void __launch_bounds__(kBlock) k_tiles(StepArgs a, int rbuf, int wbuf
                 const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
-                atomicAdd(&gw[gi], m);
-                atomicAdd(&gw[gp + gi], apx[i]);
-                atomicAdd(&gw[2 * gp + gi], apy[i]);
+                glb_add(gw + gi, m);
+                glb_add(gw + gp + gi, apx[i]);
+                glb_add(gw + 2 * gp + gi, apy[i]);
             }
         }
         __syncthreads();
@@ -168,8 +168,9 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         const SceneDev &S = a.scenes[nc.scene];
         const SceneConsts &c = S.c;
         const SceneClock clk = a.clocks[nc.scene];
-        double *g = (double *)S.grid[buf];
-        double *go = (double *)S.grid[buf ^ 1];
+        double *g = (double *)(a.grid_mem + S.grid_off[buf]);
+        double *go = (double *)(a.grid_mem + S.grid_off[buf ^ 1]);
+        double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
         const double dt = c.dt;
         for (int k = threadIdx.x; k < nc.count; k += kBlock) {
@@ -208,8 +209,8 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
                     angle += kPi;
                     angle *= c.n_ind_sub / (2 * kPi);
                     const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
-                    atomicAdd(&S.ind_force[2 * index], f0);
-                    atomicAdd(&S.ind_force[2 * index + 1], f1);
+                    atomicAdd(&force[2 * index], f0);
+                    atomicAdd(&force[2 * index + 1], f1);
                 }
             }
             if (giy <= 2) { vx = 0; vy = 0; }

@@ -30,14 +30,14 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
     v[1] = R.at(kLineVelX + 1, pt);
 #pragma unroll
     for (int m = 0; m < 4; m++) B[m] = R.at(kLineBp00 + m, pt);
-    double *gw = (double *)S.grid[wbuf];
+    glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
     const long long gp = S.gpitch;
     const int gridY = c.gridY;
     const unsigned e = p2g_point(c, x, y, v, B, F, Jp, ud, [&](int col, int row, double dm, double dpx, double dpy) {
         const long long gi = (long long)col * gridY + row;
-        atomicAdd(&gw[gi], dm);
-        atomicAdd(&gw[gp + gi], dpx);
-        atomicAdd(&gw[2 * gp + gi], dpy);
+        glb_add(gw + gi, dm);
+        glb_add(gw + gp + gi, dpx);
+        glb_add(gw + 2 * gp + gi, dpy);
     });
     if (e) atomicMax(&a.err[scene], e);
 }
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     double x, y, F[4], Jp, v[2], B[4], pp, qq;
     load_state(R, pt, x, y, F, Jp);
     unsigned e;
-    if (!g2p_point(c, (const double *)S.grid[rbuf], S.gpitch, nullptr, nullptr, 0, 0, 0, 0, x, y, v, B, F, Jp, ud,
+    if (!g2p_point(c, (const glb_d *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, nullptr, 0, 0, 0, 0, x, y, v, B, F, Jp, ud,
                    pp, qq, e)) {
         atomicMax(&a.err[scene], e);
         return;

@@ -36,13 +36,14 @@ struct SceneDev {
     long long pt_off;       // first point of the scene in the SoA
     int n_pts;              // points of the scene (after the last compaction)
     int cap_pts;            // capacity of the scene's point range
-    void *grid[2];          // two accumulation grids (3 lines of gpitch elements, or 6 for mixed)
+    long long grid_off[2];  // byte offsets of the two accumulation grids from StepArgs::grid_mem
+                            // (3 lines of gpitch elements, or 6 for mixed)
     long long gpitch;       // elements per grid line (>= ncols*gridY)
     long long cell_off;     // first entry of the scene in the cell_start array
     int ncells;             // ncols*gridY
     int n_tiles_x, n_tiles_y;
     int pad0;
-    double *ind_force;      // 2*n_ind_sub doubles
+    long long force_off;    // first of 2*n_ind_sub doubles in StepArgs::force_mem
 };
 
 // Device clock of a scene, advanced on the device once per substep so that a
@@ -73,6 +74,8 @@ struct NodeChunk {
 // Arguments common to all stepping kernels.
 struct StepArgs {
     const SceneDev *scenes;
+    char *grid_mem;           // base of every scene's grids (a kernel argument keeps it in the global space)
+    double *force_mem;        // base of every scene's indenter force bins
     SceneClock *clocks;
     void *pts;                // SoA base: kNumPtsLines lines of `pitch` elements
     long long pitch;

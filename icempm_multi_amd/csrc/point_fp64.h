@@ -10,8 +10,22 @@
 namespace icempm {
 namespace fp64 {
 
+// Explicit address spaces: LDS window accesses and global grid accesses stay
+// distinct ds_* / global_* instructions (a generic pointer that may point to
+// either compiles to slower flat_* instructions).
+typedef __attribute__((address_space(3))) double lds_d;
+typedef __attribute__((address_space(1))) double glb_d;
+
+__device__ __forceinline__ void lds_add(lds_d *p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void glb_add(glb_d *p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 constexpr int kBlock = 256;
-constexpr int kPPT = 4;          // consecutive points per thread and round
 constexpr int kMaxTX = 32;
 
 __device__ __forceinline__ long long as_ll(double d) { return __double_as_longlong(d); }
@@ -35,8 +49,8 @@ __device__ __forceinline__ int xcd_block()
 // G2P of one point: gathers v and C (kernels.cuh:344-372), advects x, updates F
 // (375-376) and runs the constitutive branch (378-391).  Returns false (and
 // sets the error byte) if the stencil leaves the grid.
-__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const double *g, long long gpitch,
-                                          const double *vwx, const double *vwy, int wc0, int wr0, int win_w,
+__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, long long gpitch,
+                                          const lds_d *vwx, const lds_d *vwy, int wc0, int wr0, int win_w,
                                           int win_h, double &x, double &y, double v[2], double B[4], double F[4],
                                           double &Jp, long long &ud, double &p_out, double &q_out, unsigned &e)
 {
@@ -55,7 +69,9 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const double *g,
     v[0] = v[1] = 0;
     B[0] = B[1] = B[2] = B[3] = 0;
     const double s4 = 4. * c.hinv;
-#pragma unroll
+    // the gather is left rolled in i: fully unrolled it keeps 18 node values
+    // live and the fused kernel drops to 2 waves per SIMD
+#pragma unroll 1
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
