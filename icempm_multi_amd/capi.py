@@ -63,6 +63,8 @@ _SIGS = {
     "icempm_partition_get_bins": (C.c_int, [_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int,
                                             C.POINTER(C.c_int32), C.c_int]),
     "icempm_partition_get_grid": (C.c_int, [_VP, C.c_int, _D, C.c_longlong]),
+    "icempm_partition_get_position_map": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                    C.POINTER(C.c_int)]),
     "icempm_partition_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     "icempm_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_VP)]),
     "icempm_batch_destroy": (C.c_int, [_VP]),
@@ -264,6 +266,11 @@ class Partition:
         out = np.zeros((3, n_nodes))
         _check(lib().icempm_partition_get_grid(self._h, buf, _dptr(out), n_nodes))
         return out
+
+    def position_map(self):
+        fb, ox, oy = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().icempm_partition_get_position_map(self._h, C.byref(fb), C.byref(ox), C.byref(oy)))
+        return fb.value, ox.value, oy.value
 
     def get_bins(self, n, n_cells):
         keys = np.zeros(max(n, 1), dtype=np.int32)

@@ -324,6 +324,12 @@ int icempm_partition_get_grid(icempm_partition *p, int buf, double *out, long lo
     return guarded([&] { return p->eng->get_grid(0, buf, out, n_nodes); });
 }
 
+int icempm_partition_get_position_map(icempm_partition *p, int *frac_bits, int *origin_x, int *origin_y)
+{
+    if (!p || !frac_bits || !origin_x || !origin_y) return bad("null argument");
+    return guarded([&] { return p->eng->get_position_map(0, frac_bits, origin_x, origin_y); });
+}
+
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen)
 {
     if (!p) return bad("null partition");

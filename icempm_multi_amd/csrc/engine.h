@@ -73,6 +73,7 @@ public:
     int get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts, int n_cells);
     int get_kernel_stats(double *ms, long long *count, int n);
     int get_grid(int scene, int buf, double *out, long long n_nodes);
+    int get_position_map(int scene, int *fb, int *ox, int *oy);
     const SceneHost &scene(int i) const { return scenes_[i]; }
     int nscenes() const { return (int)scenes_.size(); }
     long long total_points() const;

@@ -100,7 +100,8 @@ int Engine::frac_bits_for(const icempm_simparams &P, int &origin) const
     const int ncols = (scenes_.empty() ? P.GridXTotal : P.GridXTotal) + 2 * halo;
     origin = halo + 2;
     const long long span = std::max<long long>(ncols + 4, P.GridY + 4);
-    return std::min(22, 32 - bits_for(span));
+    // fx = frac * 2^-fb must stay exact in FP32 (frac < 1.5 * 2^fb): fb <= 23
+    return std::min(23, 32 - bits_for(span));
 }
 
 int Engine::add_scene(const icempm_simparams *prms, int cap_pts)
@@ -720,6 +721,18 @@ int Engine::get_error_code(uint8_t *code)
     unsigned m = 0;
     for (unsigned v : e) m = std::max(m, v);
     *code = (uint8_t)m;
+    return ICEMPM_OK;
+}
+
+int Engine::get_position_map(int scene, int *fb, int *ox, int *oy)
+{
+    if (scene < 0 || scene >= nscenes() || !scenes_[scene].have_prms)
+        return fail(ICEMPM_ERR_STATE, "get_position_map: parameters not set");
+    SceneDev d;
+    build_scene_dev(scene, d);
+    *fb = d.c.frac_bits;
+    *ox = d.c.origin;
+    *oy = d.c.origin_y;
     return ICEMPM_OK;
 }
 

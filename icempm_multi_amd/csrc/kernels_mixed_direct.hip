@@ -1,0 +1,81 @@
+// kernels_mixed_direct.hip -- unbinned mixed-precision P2G and G2P (one thread
+// per point; P2G adds every contribution with the atomic two-sum).  Separate
+// translation unit for the same compiler reason as kernels_fp64_direct.hip.
+#include <hip/hip_runtime.h>
+
+#include "point_mixed.h"
+
+namespace icempm {
+namespace mixed {
+
+__global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, int wbuf)
+{
+    const SceneDev &S = a.scenes[scene];
+    const SceneConsts &c = S.c;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.n_pts) return;
+    const long long pt = S.pt_off + i;
+    const PtRef R{(unsigned *)a.pts, a.pitch};
+    const unsigned ud = R.u(kLineUtility, pt);
+    if (ud & (unsigned)kDisabled) return;
+    const ConstsF cf = to_float(c);
+    unsigned xq, yq;
+    float D[4], Jp, v[2], C[4];
+    load_state(R, pt, xq, yq, D, Jp);
+    v[0] = R.f(kLineVelX, pt);
+    v[1] = R.f(kLineVelX + 1, pt);
+#pragma unroll
+    for (int m = 0; m < 4; m++) C[m] = R.f(kLineBp00 + m, pt);
+    glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
+    const long long gp = S.gpitch;
+    const int gridY = c.gridY;
+    const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jp, ud, [&](int col, int row, float dm, float dpx, float dpy) {
+        const long long gi = (long long)col * gridY + row;
+        twosum_add(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm, 0.f);
+        twosum_add(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx, 0.f);
+        twosum_add(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy, 0.f);
+    });
+    if (e) atomicMax(&a.err[scene], e);
+}
+
+__global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, int rbuf, int record)
+{
+    const SceneDev &S = a.scenes[scene];
+    const SceneConsts &c = S.c;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= S.n_pts) return;
+    const long long pt = S.pt_off + i;
+    const PtRef R{(unsigned *)a.pts, a.pitch};
+    unsigned ud = R.u(kLineUtility, pt);
+    if (ud & (unsigned)kDisabled) return;
+    const ConstsF cf = to_float(c);
+    unsigned xq, yq;
+    float D[4], Jp, v[2], C[4], pp, qq;
+    load_state(R, pt, xq, yq, D, Jp);
+    unsigned e;
+    if (!g2p_point(c, cf, (const glb_f *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, nullptr, 0, 0, 0, 0, xq,
+                   yq, v, C, D, Jp, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step)) {
+        atomicMax(&a.err[scene], e);
+        return;
+    }
+    const bool rec = record >= 0 ? record != 0 : (a.clocks[scene].step % c.update_every) == 0;
+    store_state(R, pt, xq, yq, v, D, C, Jp, ud, rec, pp, qq);
+}
+
+hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s)
+{
+    if (n_pts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_p2g_direct, dim3((n_pts + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, scene, wbuf);
+    return hipGetLastError();
+}
+
+hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s)
+{
+    if (n_pts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_g2p_direct, dim3((n_pts + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, scene, rbuf,
+                       record_pq);
+    return hipGetLastError();
+}
+
+}  // namespace mixed
+}  // namespace icempm

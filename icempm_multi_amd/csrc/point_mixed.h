@@ -1,0 +1,251 @@
+// point_mixed.h -- per-point G2P and P2G of the mixed-precision form.
+//
+// Point SoA: 16 lines of 4-byte elements: 0 utility (low 32 bits), 1 P, 2 Q,
+// 3 Jp_inv, 4-5 fixed-point x/y, 6-7 v, 8-11 D = F - I, 12-15 C (FP32).
+// Grid buffer: 6 FP32 lines (m, px, py as compensated hi/lo pairs); after the
+// grid update lines 2 and 4 hold the node velocity.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "physics_fp64.h"
+#include "physics_mixed.h"
+
+namespace icempm {
+namespace mixed {
+
+typedef __attribute__((address_space(3))) float lds_f;
+typedef __attribute__((address_space(3))) double lds_d;
+typedef __attribute__((address_space(1))) float glb_f;
+typedef __attribute__((address_space(1))) unsigned glb_u;
+
+constexpr int kBlock = 256;
+constexpr int kMaxTX = 32;
+constexpr int kLineMHi = 0, kLineMLo = 1, kLinePxHi = 2, kLinePxLo = 3, kLinePyHi = 4, kLinePyLo = 5;
+
+__device__ __forceinline__ void lds_add(lds_d *p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Compensated accumulation of v into the (hi, lo) pair of a node: the atomic
+// returns the old hi, so the rounding error of the stored hi + v is recovered
+// exactly (Knuth's two-sum) and added to lo.
+__device__ __forceinline__ void twosum_add(glb_f *hi, glb_f *lo, float v, float vlo)
+{
+    const float old = __hip_atomic_fetch_add(hi, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const float s = old + v;
+    const float bb = s - old;
+    const float err = (old - (s - bb)) + (v - bb);
+    const float e = err + vlo;
+    if (e != 0.f) __hip_atomic_fetch_add(lo, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void twosum_add_d(glb_f *hi, glb_f *lo, double v)
+{
+    const float h = (float)v;
+    twosum_add(hi, lo, h, (float)(v - (double)h));
+}
+
+struct PtRef {
+    unsigned *P;
+    long long pitch;
+    __device__ __forceinline__ unsigned &u(int line, long long i) const { return P[line * pitch + i]; }
+    __device__ __forceinline__ float f(int line, long long i) const { return __uint_as_float(P[line * pitch + i]); }
+    __device__ __forceinline__ void setf(int line, long long i, float v) const { P[line * pitch + i] = __float_as_uint(v); }
+};
+
+// Uniform [0, 1) from a point index and a substep (murmur3 finalizer).  Used to
+// round the fixed-point position increment stochastically: rounding to nearest
+// biases slowly varying increments of a few units every substep the same way.
+__device__ __forceinline__ float unit_hash(unsigned a, unsigned b)
+{
+    unsigned h = a * 0x9E3779B1u ^ (b * 0x85EBCA77u + 0x7F4A7C15u);
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ unsigned stencil_error(const SceneConsts &c, int bxc, int by)
+{
+    unsigned e = 0;
+    if (bxc < 0) e = 70;
+    if (by < 0) e = 71;
+    if (bxc > c.ncols - 3) e = 72;
+    if (by > c.gridY - 3) e = 73;
+    return e;
+}
+
+// The constitutive branch of partition_kernel_g2p (kernels.cuh:378-391):
+// intact points in FP32; crushed points go through the 2x2 SVD and the
+// Drucker-Prager projection in FP64.
+__device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF &cf, float D[4], float &Jp,
+                                             unsigned &ud, const float A[4], float &p_out, float &q_out)
+{
+    if (ud & (unsigned)kLiquid) {
+        Jp *= 1.f + (A[0] + A[3]) + (A[0] * A[3] - A[2] * A[1]);
+        p_out = 0; q_out = 0;
+        return;
+    }
+    float p, q;
+    compute_pq(cf, D, p, q);
+    p_out = p; q_out = q;
+    if (!(ud & (unsigned)kCrushed))
+        if (failure(cf, (short)(ud & 0xffff), p, q)) ud |= (unsigned)kCrushed;
+    if (ud & (unsigned)kCrushed) {
+        double F[4] = {1.0 + (double)D[0], (double)D[1], (double)D[2], 1.0 + (double)D[3]};
+        double Jpd = Jp, Je, pd, qd;
+        fp64::compute_pq(c, F, Je, pd, qd);
+        fp64::drucker_prager(c, F, Jpd, Je, pd, qd);
+        D[0] = (float)(F[0] - 1.0);
+        D[1] = (float)F[1];
+        D[2] = (float)F[2];
+        D[3] = (float)(F[3] - 1.0);
+        Jp = (float)Jpd;
+        p_out = (float)pd;
+        q_out = (float)qd;
+    }
+}
+
+// G2P of one point from an LDS window of node velocities (or the global
+// velocity lines if the stencil left the window).
+__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_f *g, long long gp,
+                                          const lds_f *vwx, const lds_f *vwy, int wc0, int wr0, int win_w,
+                                          int win_h, unsigned &xq, unsigned &yq, float v[2], float C[4], float D[4],
+                                          float &Jp, unsigned &ud, float &p_out, float &q_out, unsigned &e,
+                                          unsigned point, unsigned step)
+{
+    int bx, by;
+    float fx, fy;
+    base_fx_fixed(xq, cf.fb, cf.origin, bx, fx);
+    base_fx_fixed(yq, cf.fb, cf.origin_y, by, fy);
+    const int bxc = bx - c.xoff + c.halo;
+    e = stencil_error(c, bxc, by);
+    if (e) return false;
+    float wx[3], wy[3];
+    weights(fx, wx);
+    weights(fy, wy);
+    const int lx = bxc - wc0, ly = by - wr0;
+    const bool inwin = vwx && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
+    v[0] = v[1] = 0;
+    C[0] = C[1] = C[2] = C[3] = 0;
+#pragma unroll 1
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            float nvx, nvy;
+            if (inwin) {
+                const int li = (lx + i) * win_h + ly + j;
+                nvx = vwx[li];
+                nvy = vwy[li];
+            } else {
+                const long long gi = (long long)(bxc + i) * c.gridY + by + j;
+                nvx = g[kLinePxHi * gp + gi];
+                nvy = g[kLinePyHi * gp + gi];
+            }
+            const float d0 = (float)i - fx, d1 = (float)j - fy;
+            const float w = wx[i] * wy[j];
+            v[0] += w * nvx;
+            v[1] += w * nvy;
+            const float s = cf.s4 * w;
+            C[0] += s * (nvx * d0);
+            C[1] += s * (nvx * d1);
+            C[2] += s * (nvy * d0);
+            C[3] += s * (nvy * d1);
+        }
+    // x += v dt in fixed point, rounded stochastically (unbiased)
+    xq += (unsigned)__float2int_rd(v[0] * cf.dq_scale + unit_hash(point, 2 * step));
+    yq += (unsigned)__float2int_rd(v[1] * cf.dq_scale + unit_hash(point, 2 * step + 1));
+    const float A[4] = {cf.dt * C[0], cf.dt * C[1], cf.dt * C[2], cf.dt * C[3]};
+    // D <- (I + A)(I + D) - I = D + A + A D
+    const float n0 = D[0] + A[0] + (A[0] * D[0] + A[1] * D[2]);
+    const float n1 = D[1] + A[1] + (A[0] * D[1] + A[1] * D[3]);
+    const float n2 = D[2] + A[2] + (A[2] * D[0] + A[3] * D[2]);
+    const float n3 = D[3] + A[3] + (A[2] * D[1] + A[3] * D[3]);
+    D[0] = n0; D[1] = n1; D[2] = n2; D[3] = n3;
+    constitutive(c, cf, D, Jp, ud, A, p_out, q_out);
+    return true;
+}
+
+// P2G contributions of one point; calls sink(col, row, dm, dpx, dpy).
+template <class Sink>
+__device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
+                                              const float v[2], const float C[4], const float D[4], float Jp,
+                                              unsigned ud, Sink &&sink)
+{
+    const bool liquid = (ud & (unsigned)kLiquid) != 0;
+    const float mass = liquid ? cf.wmass : cf.pmass;
+    float tau[4];
+    if (liquid) water(cf, Jp, tau);
+    else kirchhoff(cf, D, tau);
+    float S[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) S[k] = mass * C[k] - cf.dt_vol_Dpinv * tau[k];
+    int bx, by;
+    float fx, fy;
+    base_fx_fixed(xq, cf.fb, cf.origin, bx, fx);
+    base_fx_fixed(yq, cf.fb, cf.origin_y, by, fy);
+    const int bxc = bx - c.xoff + c.halo;
+    const unsigned e = stencil_error(c, bxc, by);
+    if (e) return e;
+    float wx[3], wy[3];
+    weights(fx, wx);
+    weights(fy, wy);
+    const float mv0 = v[0] * mass, mv1 = v[1] * mass;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float W = wx[i] * wy[j];
+            const float d0 = ((float)i - fx) * cf.h, d1 = ((float)j - fy) * cf.h;
+            const float t0 = mv0 + (S[0] * d0 + S[1] * d1);
+            const float t1 = mv1 + (S[2] * d0 + S[3] * d1);
+            sink(bxc + i, by + j, W * mass, W * t0, W * t1);
+        }
+    return 0;
+}
+
+__device__ __forceinline__ void load_state(const PtRef &R, long long pt, unsigned &xq, unsigned &yq, float D[4],
+                                           float &Jp)
+{
+    xq = R.u(kLinePosX, pt);
+    yq = R.u(kLinePosX + 1, pt);
+#pragma unroll
+    for (int k = 0; k < 4; k++) D[k] = R.f(kLineFe00 + k, pt);
+    Jp = R.f(kLineJp, pt);
+}
+
+__device__ __forceinline__ void store_state(const PtRef &R, long long pt, unsigned xq, unsigned yq, const float v[2],
+                                            const float D[4], const float C[4], float Jp, unsigned ud, bool record,
+                                            float p, float q)
+{
+    R.u(kLinePosX, pt) = xq;
+    R.u(kLinePosX + 1, pt) = yq;
+    R.setf(kLineVelX, pt, v[0]);
+    R.setf(kLineVelX + 1, pt, v[1]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        R.setf(kLineFe00 + k, pt, D[k]);
+        R.setf(kLineBp00 + k, pt, C[k]);
+    }
+    R.setf(kLineJp, pt, Jp);
+    R.u(kLineUtility, pt) = ud;
+    if (record) {
+        R.setf(kLineP, pt, p);
+        R.setf(kLineQ, pt, q);
+    }
+}
+
+__device__ __forceinline__ int xcd_block()
+{
+    const int G = gridDim.x;
+    if ((G & 7) != 0) return blockIdx.x;
+    return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+}
+
+}  // namespace mixed
+}  // namespace icempm

@@ -234,6 +234,11 @@ int icempm_partition_get_bins(icempm_partition *p, int32_t *keys, int32_t *perm,
  * (the reference keeps the grid internal). */
 int icempm_partition_get_grid(icempm_partition *p, int buf, double *out, long long n_nodes);
 
+/* Position map of the mixed form: a coordinate is stored as the unsigned
+ * 32-bit xq with x/h = xq * 2^-frac_bits - origin_x (y: origin_y).  No
+ * reference counterpart (the reference stores FP64 positions). */
+int icempm_partition_get_position_map(icempm_partition *p, int *frac_bits, int *origin_x, int *origin_y);
+
 /* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
  * the current point set and keeps the fastest.  Results (ms per launch) of each
  * candidate go to out_ms if non-null. */

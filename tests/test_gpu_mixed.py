@@ -1,0 +1,159 @@
+"""Mixed-precision form (FP32 state, fixed-point positions, compensated grid
+sums, FP64 islands) vs the FP64 oracle, through the C-ABI.
+
+Bars: binning bit-exact on the stored (fixed-point) positions; relative
+position error ||X - X_ref|| / ||X_ref|| <= 1e-4 after the block workload's
+steps; mass conserved to 1e-9 by the compensated sums.  The other per-field
+errors are reported (printed) with loose bounds that catch gross mistakes.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import icempm_multi_amd as im
+from icempm_multi_amd import scenes
+from icempm_multi_amd.params import STATUS_CRUSHED, STATUS_DISABLED
+
+pytestmark = pytest.mark.gpu
+
+
+def make(P, h, n, precision):
+    part = im.Partition(0, 0, precision)
+    part.allocate(h.shape[1], P.GridXTotal)
+    part.update_constants(P)
+    part.transfer_points_from_soa_to_device(h, 0, n)
+    return part
+
+
+def download(part, like):
+    out = np.zeros_like(like)
+    part.transfer_from_device(out, 0)
+    return out
+
+
+def errors(out, ref, x0, n):
+    X, R = out[4:6, :n], ref[4:6, :n]
+    e = {"rel_pos": np.linalg.norm(X - R) / np.linalg.norm(R),
+         "rel_disp": np.linalg.norm(X - R) / max(np.linalg.norm(R - x0), 1e-300),
+         "max_dx_over_h": 0.0,
+         "rel_v": np.linalg.norm(out[6:8, :n] - ref[6:8, :n]) / max(np.linalg.norm(ref[6:8, :n]), 1e-300)}
+    Fi = ref[8:12, :n] - np.array([1, 0, 0, 1.0])[:, None]
+    e["rel_F_minus_I"] = np.linalg.norm(out[8:12, :n] - ref[8:12, :n]) / max(np.linalg.norm(Fi), 1e-300)
+    e["rel_Jp"] = np.linalg.norm(out[3, :n] - ref[3, :n]) / np.linalg.norm(ref[3, :n])
+    cr_o = (out[0, :n].view(np.int64) & STATUS_CRUSHED) != 0
+    cr_r = (ref[0, :n].view(np.int64) & STATUS_CRUSHED) != 0
+    e["crushed_mismatch"] = int(np.count_nonzero(cr_o != cr_r))
+    e["crushed_ref"] = int(np.count_nonzero(cr_r))
+    return e
+
+
+def test_roundtrip_quantization():
+    P, h, n = scenes.block_scene(nx=40, ny=10, grid=(64, 32), n_grains=30)
+    r = np.random.default_rng(1)
+    h[6:8, :n] = r.normal(size=(2, n))
+    h[8:12, :n] += 1e-4 * r.normal(size=(4, n))
+    h[12:16, :n] = r.normal(size=(4, n))
+    part = make(P, h, n, im.PRECISION_MIXED)
+    out = download(part, h)
+    fb, ox, oy = part.position_map()
+    assert 16 <= fb <= 23 and ox == P.GridHaloSize + 2 and oy == 2
+    assert np.abs(out[4:6, :n] - h[4:6, :n]).max() <= 0.5001 * 2.0 ** -fb * P.cellsize
+    assert np.allclose(out[6:8, :n], h[6:8, :n], rtol=6e-8)
+    # F is stored as F - I in FP32: absolute error ~ 6e-8 of the strain
+    Fd = h[8:12, :n] - np.array([1, 0, 0, 1.0])[:, None]
+    assert np.abs(out[8:12, :n] - h[8:12, :n]).max() <= 6e-8 * np.abs(Fd).max()
+    assert np.array_equal(out[0, :n].view(np.int64), h[0, :n].view(np.int64))
+
+
+def test_mixed_binning_bit_exact():
+    P, h, n = scenes.block_scene(seed=0x1CE0900, nx=48, ny=12, grid=(96, 48), n_grains=40)
+    r = np.random.default_rng(3)
+    perm = r.permutation(n)
+    h[:, :n] = h[:, perm]
+    ud = h[0, :n].view(np.int64)
+    ud[r.uniform(size=n) < 0.03] |= STATUS_DISABLED
+    part = make(P, h, n, im.PRECISION_MIXED)
+    fb, ox, oy = part.position_map()
+    stored = download(part, h)
+    xq = np.rint((stored[4, :n] / P.cellsize + ox) * 2.0 ** fb).astype(np.int64)
+    yq = np.rint((stored[5, :n] / P.cellsize + oy) * 2.0 ** fb).astype(np.int64)
+    part.sort_points()
+    active = np.nonzero((stored[0, :n].view(np.int64) & STATUS_DISABLED) == 0)[0]
+    keys = np.array([O.cell_key_fixed(xq[i], yq[i], fb, ox, oy, P.GridY) for i in range(n)])
+    order = active[np.argsort(keys[active], kind="stable")]
+    m = len(order)
+    assert part.nPts_partition == m
+    ncells = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY
+    k_gpu, p_gpu, counts = part.get_bins(m, ncells)
+    assert np.array_equal(p_gpu, order)
+    assert np.array_equal(k_gpu, keys[order])
+    assert np.array_equal(counts, np.bincount(keys[order] + P.GridHaloSize * P.GridY, minlength=ncells))
+
+
+def test_compensated_grid_mass():
+    P, h, n = scenes.block_scene(seed=0x1CE0901)
+    part = make(P, h, n, im.PRECISION_MIXED)
+    part.sort_points()
+    nn = O.n_nodes(P)
+    part.reset_grid()
+    part.p2g()
+    g = part.get_grid(0, nn)
+    M = n * P.ParticleMass
+    assert abs(g[0].sum() - M) <= 1e-9 * M
+    ref = h.copy()
+    O.sort(P, ref, n)
+    gr, _ = O.p2g(P, ref, n)
+    # FP32 weights: each node mass within a few FP32 ulps of the FP64 sum
+    assert np.abs(g[0] - gr[0]).max() <= 5e-7 * gr[0].max()
+
+
+@pytest.mark.parametrize("flags", [0, im.STEP_GRAPH, im.STEP_UNFUSED, im.STEP_DIRECT_P2G])
+def test_block_accuracy_vs_oracle(flags):
+    """The 40 x 10 m block (25.6k points, E = 9e9, h = 0.25, dt = 1e-5) in the
+    mixed form against the FP64 oracle after 1000 substeps."""
+    P, h, n = scenes.block_scene()
+    ref = h.copy()
+    part = make(P, h, n, im.PRECISION_MIXED)
+    if not flags & im.STEP_DIRECT_P2G:
+        part.sort_points()
+        O.sort(P, ref, n)
+    x0 = ref[4:6, :n].copy()
+    steps = 1000
+    for k in range(0, steps, 500):
+        part.step(500, flags)
+    O.substeps(P, ref, n, steps, 0.0, 0)
+    out = download(part, h)
+    assert part.error_code == 0
+    e = errors(out, ref, x0, n)
+    print("mixed vs FP64 oracle after", steps, "substeps:", e)
+    assert e["rel_pos"] <= 1e-4
+    assert e["rel_disp"] <= 5e-2
+    assert e["rel_v"] <= 5e-2
+    assert e["crushed_mismatch"] <= max(5, 0.01 * n)
+
+
+def test_mixed_error_code():
+    P, h, n = scenes.block_scene(nx=40, ny=10, grid=(64, 32), n_grains=30)
+    h[5, 3] = (P.GridY - 1.4) * P.cellsize
+    part = make(P, h, n, im.PRECISION_MIXED)
+    part.step(1, im.STEP_DIRECT_P2G)
+    assert part.error_code == 73
+
+
+def test_mixed_indenter_sheet_runs():
+    P, h, n = scenes.sheet_scene(nx=400, ny=40, grid=(512, 128), h=0.05, n_grains=100)
+    ref = h.copy()
+    part = make(P, h, n, im.PRECISION_MIXED)
+    part.sort_points()
+    O.sort(P, ref, n)
+    x0 = ref[4:6, :n].copy()
+    part.step(300, im.STEP_GRAPH)
+    f = part.indenter_force()
+    _, _, fr, _ = O.substeps(P, ref, n, 300, 0.0, 0)
+    out = download(part, h)
+    e = errors(out, ref, x0, n)
+    print("sheet with indenter, 300 substeps:", e)
+    assert part.error_code == 0
+    assert e["rel_pos"] <= 1e-4
+    tot, totr = f.reshape(-1, 2).sum(0), fr.reshape(-1, 2).sum(0)
+    assert np.abs(tot - totr).max() <= 0.05 * np.abs(totr).max()
