@@ -58,6 +58,55 @@ def cpu_baseline(P, hssoa, n, substeps):
                       f"{dt:.1f} s wall"}
 
 
+def calibration_copy():
+    """A plain device-to-device copy of 1 GiB (reads 1 GiB, writes 1 GiB): the
+    reference launch for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def accuracy(im, P, hssoa, n, steps):
+    """Mixed vs FP64 GPU stepping from the same state, points matched by
+    position (KD-tree): relative position error and per-field errors."""
+    from scipy.spatial import cKDTree
+    outs = {}
+    for name, prec in (("fp64", im.PRECISION_FP64), ("mixed", im.PRECISION_MIXED)):
+        part = im.Partition(0, 0, prec)
+        part.allocate(hssoa.shape[1], P.GridXTotal)
+        part.update_constants(P)
+        part.transfer_points_from_soa_to_device(hssoa, 0, n)
+        part.sort_points()
+        part.step(steps, im.STEP_GRAPH)
+        o = np.zeros_like(hssoa)
+        part.transfer_from_device(o, 0)
+        outs[name] = o[:, :n]
+        part.close()
+    ref, mx = outs["fp64"], outs["mixed"]
+    _, idx = cKDTree(ref[4:6].T).query(mx[4:6].T, workers=-1)
+    ok = len(np.unique(idx)) == n
+    mxr = np.empty_like(mx)
+    mxr[:, idx] = mx
+    x0 = np.empty((2, n))
+    _, i0 = cKDTree(hssoa[4:6, :n].T).query(ref[4:6].T, workers=-1)
+    x0[:] = hssoa[4:6, :n][:, i0]
+    d = mxr[4:6] - ref[4:6]
+    Fi = ref[8:12] - np.array([1, 0, 0, 1.0])[:, None]
+    crushed = lambda a: (a[0].view(np.int64) & 0x10000) != 0
+    return {"substeps": steps, "matched_one_to_one": bool(ok),
+            "rel_pos": float(np.linalg.norm(d) / np.linalg.norm(ref[4:6])),
+            "rel_disp": float(np.linalg.norm(d) / max(np.linalg.norm(ref[4:6] - x0), 1e-300)),
+            "max_abs_dx_m": float(np.abs(d).max()),
+            "rel_v": float(np.linalg.norm(mxr[6:8] - ref[6:8]) / max(np.linalg.norm(ref[6:8]), 1e-300)),
+            "rel_F_minus_I": float(np.linalg.norm(mxr[8:12] - ref[8:12]) / max(np.linalg.norm(Fi), 1e-300)),
+            "rel_Jp": float(np.linalg.norm(mxr[3] - ref[3]) / np.linalg.norm(ref[3])),
+            "crushed_ref": int(crushed(ref).sum()), "crushed_mismatch": int((crushed(ref) != crushed(mxr)).sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +117,10 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay frames as hipGraphs (no per-kernel events)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
+    ap.add_argument("--accuracy-steps", type=int, default=1000, help="substeps of the mixed vs FP64 comparison")
+    ap.add_argument("--calib-copy", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
+    ap.add_argument("--tile", default=None, help="TXxTY tile shape instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU stepping (x-slab halo exchange) "
@@ -77,75 +130,97 @@ def main():
     import icempm_multi_amd as im
     from icempm_multi_amd import scenes
 
+    if args.calib_copy:
+        calibration_copy()
+
     nx = int(round(4000 * args.scale))
     t0 = time.perf_counter()
     P, hssoa, n = scenes.sheet_scene(nx=nx)
     log(f"scene: {n} points, grid {P.GridXTotal}x{P.GridY}, h={P.cellsize}, setup {time.perf_counter() - t0:.1f}s")
-    prec = im.PRECISION_MIXED if args.precision == "mixed" else im.PRECISION_FP64
-    part = im.Partition(0, 0, prec)
-    part.allocate(hssoa.shape[1], P.GridXTotal)
-    part.update_constants(P)
-    part.transfer_points_from_soa_to_device(hssoa, 0, n)
-    part.sort_points()
-    chosen, tune_ms = part.autotune(3)
-    log(f"autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:8]]}")
-
-    frame = P.UpdateEveryNthStep
-    flags_timed = im.STEP_TIMINGS if not args.graph else im.STEP_GRAPH
-
-    def run(k, flags):
-        done = 0
-        while done < k:
-            m = min(frame - (part.clock[1] % frame), k - done)
-            resort = im.STEP_RESORT if part.clock[1] % frame == 0 and part.clock[1] > 0 else 0
-            part.step(m, flags | resort)
-            done += m
-
-    run(args.warmup, flags_timed)
-    part.synchronize()
-    part.reset_indenter_force_accumulator()          # also resets the kernel statistics
-    t0 = time.perf_counter()
-    run(args.steps, flags_timed)
-    part.synchronize()
-    wall = time.perf_counter() - t0
-    if part.error_code:
-        raise RuntimeError(f"device error code {part.error_code}")
-    value = n * args.steps / wall / 1e6
-    stats = part.kernel_stats()
-    log("kernel stats (ms, launches):", {k: (round(v[0], 2), v[1]) for k, v in stats.items()})
-
-    # nodes touched by the points (for the grid part of the algorithmic bytes)
     nn = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY
-    part.reset_grid()
-    part.p2g()
-    g_act = int(np.count_nonzero(part.get_grid(0, nn)[0]))
-    B = BYTES[args.precision]
-    fused_ms, fused_n = stats["fused_g2p_p2g"]
-    roofline = None
-    if fused_n > 0:
-        t_launch = fused_ms / fused_n * 1e-3
-        algo = n * B["fused_point"] + g_act * B["fused_node"]
-        achieved = algo / t_launch / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                    "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
-                    "kernel": "k_tiles<fused G2P+P2G>", "avg_launch_us": round(t_launch * 1e6, 2),
-                    "algo_bytes_per_launch": algo}
-    step_bytes = n * B["step_point"] + P.GridXTotal * P.GridY * B["step_node"]
-    step_s = wall / args.steps
+
+    def timed(precision_name):
+        prec = im.PRECISION_MIXED if precision_name == "mixed" else im.PRECISION_FP64
+        part = im.Partition(0, 0, prec)
+        part.allocate(hssoa.shape[1], P.GridXTotal)
+        part.update_constants(P)
+        part.transfer_points_from_soa_to_device(hssoa, 0, n)
+        part.sort_points()
+        if args.tile:
+            tx, ty = (int(v) for v in args.tile.lower().split("x"))
+            part.set_tile_shape(tx, ty)
+            chosen = args.tile
+        else:
+            chosen, tune_ms = part.autotune(3)
+            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:10]]}")
+        frame = P.UpdateEveryNthStep
+        flags_timed = im.STEP_TIMINGS if not args.graph else im.STEP_GRAPH
+
+        def run(k, flags):
+            done = 0
+            while done < k:
+                m = min(frame - (part.clock[1] % frame), k - done)
+                resort = im.STEP_RESORT if part.clock[1] % frame == 0 and part.clock[1] > 0 else 0
+                part.step(m, flags | resort)
+                done += m
+
+        run(args.warmup, flags_timed)
+        part.synchronize()
+        part.reset_indenter_force_accumulator()          # also resets the kernel statistics
+        t0 = time.perf_counter()
+        run(args.steps, flags_timed)
+        part.synchronize()
+        wall = time.perf_counter() - t0
+        if part.error_code:
+            raise RuntimeError(f"device error code {part.error_code}")
+        stats = part.kernel_stats()
+        log(f"[{precision_name}] kernel stats (ms, launches):", {k: (round(v[0], 2), v[1]) for k, v in stats.items()})
+        part.reset_grid()
+        part.p2g()
+        g_act = int(np.count_nonzero(part.get_grid(0, nn)[0]))
+        B = BYTES[precision_name]
+        fused_ms, fused_n = stats["fused_g2p_p2g"]
+        roofline = None
+        if fused_n > 0:
+            t_launch = fused_ms / fused_n * 1e-3
+            algo = n * B["fused_point"] + g_act * B["fused_node"]
+            achieved = algo / t_launch / 1e9
+            roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                        "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
+                        "kernel": f"{precision_name}::k_tiles<3> (fused G2P+P2G)",
+                        "avg_launch_us": round(t_launch * 1e6, 2), "algo_bytes_per_launch": algo,
+                        "bytes_per_point": B["fused_point"], "bytes_per_node": B["fused_node"],
+                        "touched_nodes": g_act}
+        step_s = wall / args.steps
+        step_bytes = n * B["step_point"] + P.GridXTotal * P.GridY * B["step_node"]
+        part.close()
+        return {"value": n * args.steps / wall / 1e6, "ms_per_step": step_s * 1e3, "roofline": roofline,
+                "hbm_model_GBps": step_bytes / step_s / 1e9, "hbm_model_frac": step_bytes / step_s / PEAK_HBM,
+                "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]}, "tile": chosen}
+
+    primary = timed(args.precision)
     out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "M particle-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4),
+        "metric": METRIC, "value": round(primary["value"], 2), "unit": "M particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(primary["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"ice sheet {nx * 0.05:g} x 12.5 m, grid {P.GridXTotal}x{P.GridY}, h=0.05 m, "
                                f"{n} points (4/cell jittered), cylindrical indenter D=1 m at 2 m/s, dt=1e-5 s",
                    "points": n, "grid": [P.GridXTotal, P.GridY], "precision": args.precision,
-                   "tile": chosen, "mode": "graph" if args.graph else "eager+events"},
-        "hbm_model_GBps": round(step_bytes / step_s / 1e9, 1),
-        "hbm_model_frac": round(step_bytes / step_s / PEAK_HBM, 4),
-        "roofline": roofline,
-        "kernel_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in stats.items() if v[1]},
+                   "tile": primary["tile"], "mode": "graph" if args.graph else "eager+events"},
+        "hbm_model_GBps": round(primary["hbm_model_GBps"], 1),
+        "hbm_model_frac": round(primary["hbm_model_frac"], 4),
+        "roofline": primary["roofline"],
+        "kernel_ms": primary["kernel_ms"],
     }
+    if args.precision == "fp64" and not args.no_mixed:
+        mx = timed("mixed")
+        acc = accuracy(im, P, hssoa, n, args.accuracy_steps) if args.accuracy_steps > 0 else None
+        out["mixed"] = {"value": round(mx["value"], 2), "unit": "M particle-steps/s",
+                        "speedup_vs_fp64": round(mx["value"] / primary["value"], 3),
+                        "ms_per_step": round(mx["ms_per_step"], 4), "roofline": mx["roofline"],
+                        "hbm_model_frac": round(mx["hbm_model_frac"], 4), "kernel_ms": mx["kernel_ms"],
+                        "accuracy_vs_fp64": acc}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, hssoa, n, args.cpu_substeps)
     else:

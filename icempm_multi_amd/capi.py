@@ -65,6 +65,7 @@ _SIGS = {
     "icempm_partition_get_grid": (C.c_int, [_VP, C.c_int, _D, C.c_longlong]),
     "icempm_partition_get_position_map": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                     C.POINTER(C.c_int)]),
+    "icempm_partition_set_tile_shape": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "icempm_partition_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     "icempm_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_VP)]),
     "icempm_batch_destroy": (C.c_int, [_VP]),
@@ -221,6 +222,9 @@ class Partition:
 
     def sort_points(self):
         _check(lib().icempm_partition_sort_points(self._h))
+
+    def set_tile_shape(self, tx, ty, margin=1):
+        _check(lib().icempm_partition_set_tile_shape(self._h, tx, ty, margin))
 
     def autotune(self, n_trials=5):
         out = (C.c_float * 16)()

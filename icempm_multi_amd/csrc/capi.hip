@@ -330,6 +330,12 @@ int icempm_partition_get_position_map(icempm_partition *p, int *frac_bits, int *
     return guarded([&] { return p->eng->get_position_map(0, frac_bits, origin_x, origin_y); });
 }
 
+int icempm_partition_set_tile_shape(icempm_partition *p, int tx, int ty, int margin)
+{
+    if (!p) return bad("null partition");
+    return guarded([&] { return p->eng->set_tile_shape(tx, ty, margin); });
+}
+
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen)
 {
     if (!p) return bad("null partition");

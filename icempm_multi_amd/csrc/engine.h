@@ -18,6 +18,8 @@
 
 namespace icempm {
 
+constexpr int kUpdateChunk = 1024;  // nodes per grid-update work item (4 per thread, persistent workgroups)
+
 enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kNumCls };
 
 struct SceneHost {
@@ -83,6 +85,7 @@ public:
     int tx() const { return tx_; }
     int ty() const { return ty_; }
     int set_tile_shape(int tx, int ty, int margin);
+    void set_lane_spread(int v) { lane_spread_ = v ? 1 : 0; invalidate_graphs(); }
 
 private:
     int fail(int code, const std::string &msg);
@@ -130,7 +133,8 @@ private:
     int *d_n_tiles_ = nullptr;
     long long max_tiles_ = 0;
     int n_live_tiles_ = 0;
-    int tx_ = 4, ty_ = 32, margin_ = 1;
+    int tx_ = 8, ty_ = 32, margin_ = 1;
+    int lane_spread_ = 1;
     std::vector<SceneDev> host_dev_;
 
     // timings

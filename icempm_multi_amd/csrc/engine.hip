@@ -340,7 +340,7 @@ int Engine::commit()
     for (int i = 0; i < nscenes(); i++) {
         const icempm_simparams &P = scenes_[i].prms;
         const int nodes = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY;
-        for (int n0 = 0; n0 < nodes; n0 += 1024) chunks.push_back({i, n0, std::min(1024, nodes - n0), 0});
+        for (int n0 = 0; n0 < nodes; n0 += kUpdateChunk) chunks.push_back({i, n0, std::min(kUpdateChunk, nodes - n0), 0});
     }
     if (d_chunks_) hipFree(d_chunks_);
     n_chunks_ = (int)chunks.size();
@@ -418,6 +418,7 @@ StepArgs Engine::args() const
     a.margin = margin_;
     a.win_w = tx_ + 2 + 2 * margin_;
     a.win_h = ty_ + 2 + 2 * margin_;
+    a.lane_spread = lane_spread_;
     return a;
 }
 
@@ -659,7 +660,7 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
         for (int i = 0; i < nscenes(); i++) {
             const icempm_simparams &P = scenes_[i].prms;
             const long long nodes = (long long)(P.GridXTotal + 2 * P.GridHaloSize) * P.GridY;
-            const int nc = (int)((nodes + 1023) / 1024);
+            const int nc = (int)((nodes + kUpdateChunk - 1) / kUpdateChunk);
             if (i == scene) { first = d_chunks_ + k; nchunks = nc; }
             k += nc;
         }
@@ -942,17 +943,20 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
 {
     RC(ensure_committed());
     if (!bins_valid_) RC(sort());
-    static const int cand[][2] = {{2, 32}, {4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {16, 16}, {2, 64}};
+    // candidates: tile shape (tx, ty) x lane spread
+    static const int cand[][3] = {{4, 32, 0}, {4, 64, 0}, {8, 16, 0}, {8, 32, 0}, {16, 16, 0},
+                                  {4, 32, 1}, {4, 64, 1}, {8, 16, 1}, {8, 32, 1}, {16, 16, 1}};
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
     // backup: points into the sort buffer, clocks and forces on the host
     HIPCHK(hipMemcpyAsync(pts_[1], pts_[0], pts_bytes, hipMemcpyDeviceToDevice, stream_));
     std::vector<SceneHost> saved = scenes_;
-    const int tx0 = tx_, ty0 = ty_;
+    const int tx0 = tx_, ty0 = ty_, ls0 = lane_spread_;
     float best = 1e30f;
     int best_i = -1;
     n_trials = std::max(1, n_trials);
     for (int c = 0; c < ncand; c++) {
+        lane_spread_ = cand[c][2];
         if (set_tile_shape(cand[c][0], cand[c][1], margin_) != ICEMPM_OK) {
             if (c < n_out && out_ms) out_ms[c] = -1;
             continue;
@@ -977,8 +981,13 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     }
     RC(push_clocks());
     RC(reset_indenter_force());
-    if (best_i < 0) RC(set_tile_shape(tx0, ty0, margin_));
-    else RC(set_tile_shape(cand[best_i][0], cand[best_i][1], margin_));
+    if (best_i < 0) {
+        lane_spread_ = ls0;
+        RC(set_tile_shape(tx0, ty0, margin_));
+    } else {
+        lane_spread_ = cand[best_i][2];
+        RC(set_tile_shape(cand[best_i][0], cand[best_i][1], margin_));
+    }
     if (chosen) *chosen = best_i;
     HIPCHK(hipStreamSynchronize(stream_));
     return ICEMPM_OK;

@@ -14,8 +14,9 @@
 // point runs of a tile come from the cell_start table of the last sort; a
 // point that drifted out of the window since then falls back to global memory.
 //
-// Built with -ffp-contract=off: every product/sum is rounded separately, as in
-// the FP64 oracle; the only source of difference is the order of the atomic sums.
+// Products and sums may contract to FMA (as the reference's nvcc build does by
+// default); the FP64 oracle rounds each operation, so the two agree to a few
+// ulps per operation plus the reordering of the atomic sums.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -98,8 +99,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         // consecutive points of a run, so every SoA access is coalesced
         unsigned err = 0;
         int cidx = 0;
+        // lane_spread: the lanes of one atomic instruction then hold points of
+        // different cells (4 points per cell), so their LDS targets differ
+        const int lane = tid & 63;
+        const int koff = (tid & ~63) + (a.lane_spread ? ((lane & 15) << 2) + (lane >> 4) : lane);
 #pragma unroll 1
-        for (int k = tid; k < total; k += kBlock) {
+        for (int k0 = 0; k0 < total; k0 += kBlock) {
+            const int k = k0 + koff;
+            if (k >= total) continue;
+            if (k < run_pre[cidx]) cidx = 0;
             while (k >= run_pre[cidx + 1]) cidx++;
             const long long pt = run_start[cidx] + (k - run_pre[cidx]);
             long long ud = as_ll(R.at(kLineUtility, pt));

@@ -3,7 +3,6 @@
 // gpu_partition.cu:230-293).  Used before the first sort and as the baseline.
 // Kept in its own translation unit: compiled together with the binned kernels
 // it trips an inliner crash of the ROCm 7.2 device compiler.
-// Built with -ffp-contract=off (see kernels_fp64.hip).
 #include <hip/hip_runtime.h>
 
 #include "point_fp64.h"

@@ -83,8 +83,15 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 
         unsigned err = 0;
         int cidx = 0;
+        // lane_spread: the lanes of one atomic instruction then hold points of
+        // different cells (4 points per cell), so their LDS targets differ
+        const int lane = tid & 63;
+        const int koff = (tid & ~63) + (a.lane_spread ? ((lane & 15) << 2) + (lane >> 4) : lane);
 #pragma unroll 1
-        for (int k = tid; k < total; k += kBlock) {
+        for (int k0 = 0; k0 < total; k0 += kBlock) {
+            const int k = k0 + koff;
+            if (k >= total) continue;
+            if (k < run_pre[cidx]) cidx = 0;
             while (k >= run_pre[cidx + 1]) cidx++;
             const long long pt = run_start[cidx] + (k - run_pre[cidx]);
             unsigned ud = R.u(kLineUtility, pt);

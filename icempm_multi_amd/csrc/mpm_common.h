@@ -88,6 +88,7 @@ struct StepArgs {
     unsigned *err;            // per-scene error byte (max over points)
     int tx, ty, margin;       // tile shape and drift margin of the LDS windows
     int win_w, win_h;         // tx+2+2*margin, ty+2+2*margin
+    int lane_spread;          // 1: lane l of a wave takes point (l%16)*4 + l/16 of its 64-point group
 };
 
 }  // namespace icempm

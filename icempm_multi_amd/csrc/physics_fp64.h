@@ -2,10 +2,7 @@
 // reference's kernels (simulation/kernels.cuh, helper_math.cuh, givens.cuh).
 //
 // Matrices are row-major double[4] (a00, a01, a10, a11).  Operation order
-// follows the reference's Eigen expressions; the translation unit that
-// includes this header is compiled with -ffp-contract=off so that nothing is
-// fused and the FP64 path tracks the reference arithmetic operation by
-// operation.
+// follows the reference's Eigen expressions.
 #pragma once
 
 #include <hip/hip_runtime.h>

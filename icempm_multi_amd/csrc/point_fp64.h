@@ -31,10 +31,17 @@ constexpr int kMaxTX = 32;
 __device__ __forceinline__ long long as_ll(double d) { return __double_as_longlong(d); }
 __device__ __forceinline__ double as_d(long long v) { return __longlong_as_double(v); }
 
+// SoA access: a uniform (scalar) line base plus a 32-bit byte offset, so loads
+// and stores use the saddr + voffset form (points per launch < 2^29).
 struct PtRef {
     double *P;
     long long pitch;
-    __device__ __forceinline__ double &at(int line, long long i) const { return P[line * pitch + i]; }
+    __device__ __forceinline__ glb_d &at(int line, long long i) const
+    {
+        typedef __attribute__((address_space(1))) char glb_c;
+        glb_c *base = (glb_c *)(P + line * pitch);
+        return *(glb_d *)(base + ((unsigned)i << 3));
+    }
 };
 
 // Logical block index that keeps consecutive work items on one XCD: workgroups

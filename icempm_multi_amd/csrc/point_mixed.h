@@ -48,12 +48,18 @@ __device__ __forceinline__ void twosum_add_d(glb_f *hi, glb_f *lo, double v)
     twosum_add(hi, lo, h, (float)(v - (double)h));
 }
 
+// SoA access: uniform line base + 32-bit byte offset (saddr + voffset form).
 struct PtRef {
     unsigned *P;
     long long pitch;
-    __device__ __forceinline__ unsigned &u(int line, long long i) const { return P[line * pitch + i]; }
-    __device__ __forceinline__ float f(int line, long long i) const { return __uint_as_float(P[line * pitch + i]); }
-    __device__ __forceinline__ void setf(int line, long long i, float v) const { P[line * pitch + i] = __float_as_uint(v); }
+    __device__ __forceinline__ glb_u &u(int line, long long i) const
+    {
+        typedef __attribute__((address_space(1))) char glb_c;
+        glb_c *base = (glb_c *)(P + line * pitch);
+        return *(glb_u *)(base + ((unsigned)i << 2));
+    }
+    __device__ __forceinline__ float f(int line, long long i) const { return __uint_as_float(u(line, i)); }
+    __device__ __forceinline__ void setf(int line, long long i, float v) const { u(line, i) = __float_as_uint(v); }
 };
 
 // Uniform [0, 1) from a point index and a substep (murmur3 finalizer).  Used to

@@ -239,6 +239,11 @@ int icempm_partition_get_grid(icempm_partition *p, int buf, double *out, long lo
  * reference counterpart (the reference stores FP64 positions). */
 int icempm_partition_get_position_map(icempm_partition *p, int *frac_bits, int *origin_x, int *origin_y);
 
+/* Tile shape of the binned kernels: tx x ty cells per workgroup tile, LDS
+ * windows widened by `margin` cells for points that drifted since the last
+ * sort.  The autotuner sets it; callers may pin it. */
+int icempm_partition_set_tile_shape(icempm_partition *p, int tx, int ty, int margin);
+
 /* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
  * the current point set and keeps the fastest.  Results (ms per launch) of each
  * candidate go to out_ms if non-null. */
