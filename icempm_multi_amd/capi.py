@@ -77,6 +77,7 @@ _SIGS = {
     "icempm_batch_get_error_code": (C.c_int, [_VP, C.POINTER(C.c_uint8)]),
     "icempm_batch_get_scene_count": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "icempm_batch_last_step_ms": (C.c_int, [_VP, C.POINTER(C.c_float)]),
+    "icempm_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -342,6 +343,12 @@ class Batch:
         n, t = C.c_int(), C.c_longlong()
         _check(lib().icempm_batch_get_scene_count(self._h, C.byref(n), C.byref(t)))
         return n.value, t.value
+
+    def autotune(self, n_trials=3):
+        out = (C.c_float * 16)()
+        chosen = C.c_int(-1)
+        _check(lib().icempm_batch_autotune(self._h, n_trials, out, 16, C.byref(chosen)))
+        return chosen.value, list(out)
 
     def last_step_ms(self):
         v = C.c_float()

@@ -452,6 +452,16 @@ int icempm_batch_get_scene_count(icempm_batch *b, int *n_scenes, long long *n_po
     return ICEMPM_OK;
 }
 
+int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_out, int *chosen)
+{
+    if (!b) return bad("null batch");
+    if (!b->committed) {
+        icempm::set_last_error("batch not committed");
+        return ICEMPM_ERR_STATE;
+    }
+    return guarded([&] { return b->eng->autotune(n_trials, out_ms, n_out, chosen); });
+}
+
 int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep)
 {
     if (!b || !ms_per_substep) return bad("null argument");

@@ -268,6 +268,8 @@ int icempm_batch_transfer_from_device(icempm_batch *b, int scene_id, double *hss
 int icempm_batch_get_indenter_force(icempm_batch *b, int scene_id, double *out, int n_doubles);
 int icempm_batch_get_error_code(icempm_batch *b, uint8_t *code);
 int icempm_batch_get_scene_count(icempm_batch *b, int *n_scenes, long long *n_points_total);
+/* The autotuner of icempm_partition_autotune over all scenes of the batch. */
+int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_out, int *chosen);
 /* Average device time per substep (ms) of the last icempm_batch_step call. */
 int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep);
 
