@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 
     const int WN = a.win_w * a.win_h;
     lds_d *lds = (lds_d *)lds_raw;
-    lds_d *vwx = lds, *vwy = lds + WN;
+    lds_d *vw = lds;                       // 2*WN: (vx, vy) per window node
     lds_d *am = lds + (DO_G2P ? 2 * WN : 0), *apx = am + WN, *apy = apx + WN;
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
@@ -82,8 +82,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                     vx = gr[gp + gi];
                     vy = gr[2 * gp + gi];
                 }
-                vwx[i] = vx;
-                vwy[i] = vy;
+                vw[2 * i] = vx;
+                vw[2 * i + 1] = vy;
             }
             if (DO_P2G) { am[i] = 0; apx[i] = 0; apy[i] = 0; }
         }
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             if (DO_G2P) {
                 double pp, qq;
                 unsigned e;
-                if (!g2p_point(c, gr, gp, vwx, vwy, wc0, wr0, a.win_w, a.win_h, x, y, v, B, F, Jp, ud, pp, qq, e)) {
+                if (!g2p_point(c, gr, gp, vw, wc0, wr0, a.win_w, a.win_h, x, y, v, B, F, Jp, ud, pp, qq, e)) {
                     err = max(err, e);
                     continue;
                 }

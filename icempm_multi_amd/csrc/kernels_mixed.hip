@@ -29,7 +29,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
     const int WN = a.win_w * a.win_h;
     // accumulators (FP64) first, then the FP32 velocity window
     lds_d *am = (lds_d *)lds_raw, *apx = am + WN, *apy = apx + WN;
-    lds_f *vwx = (lds_f *)(am + (DO_P2G ? 3 * WN : 0)), *vwy = vwx + WN;
+    lds_f *vw = (lds_f *)(am + (DO_P2G ? 3 * WN : 0));   // 2*WN: (vx, vy) per window node
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(unsigned *)a.pts, a.pitch};
@@ -68,8 +68,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                     vx = gr[kLinePxHi * gp + gi];
                     vy = gr[kLinePyHi * gp + gi];
                 }
-                vwx[i] = vx;
-                vwy[i] = vy;
+                vw[2 * i] = vx;
+                vw[2 * i + 1] = vy;
             }
             if (DO_P2G) { am[i] = 0; apx[i] = 0; apy[i] = 0; }
         }
@@ -102,7 +102,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             if (DO_G2P) {
                 float pp, qq;
                 unsigned e;
-                if (!g2p_point(c, cf, gr, gp, vwx, vwy, wc0, wr0, a.win_w, a.win_h, xq, yq, v, C, D, Jp, ud, pp, qq,
+                if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, a.win_w, a.win_h, xq, yq, v, C, D, Jp, ud, pp, qq,
                                e, (unsigned)pt, (unsigned)step)) {
                     err = max(err, e);
                     continue;

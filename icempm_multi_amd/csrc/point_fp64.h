@@ -57,7 +57,7 @@ __device__ __forceinline__ int xcd_block()
 // (375-376) and runs the constitutive branch (378-391).  Returns false (and
 // sets the error byte) if the stencil leaves the grid.
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, long long gpitch,
-                                          const lds_d *vwx, const lds_d *vwy, int wc0, int wr0, int win_w,
+                                          const lds_d *vw, int wc0, int wr0, int win_w,
                                           int win_h, double &x, double &y, double v[2], double B[4], double F[4],
                                           double &Jp, long long &ud, double &p_out, double &q_out, unsigned &e)
 {
@@ -72,7 +72,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     weights(fx, wx);
     weights(fy, wy);
     const int lx = bxc - wc0, ly = by - wr0;
-    const bool inwin = vwx && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
+    const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
     v[0] = v[1] = 0;
     B[0] = B[1] = B[2] = B[3] = 0;
     const double s4 = 4. * c.hinv;
@@ -85,8 +85,8 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
             double nvx, nvy;
             if (inwin) {
                 const int li = (lx + i) * win_h + ly + j;
-                nvx = vwx[li];
-                nvy = vwy[li];
+                nvx = vw[2 * li];           // (vx, vy) interleaved: one 128-bit LDS read
+                nvy = vw[2 * li + 1];
             } else {
                 const long long gi = (long long)(bxc + i) * c.gridY + by + j;
                 nvx = g[gpitch + gi];

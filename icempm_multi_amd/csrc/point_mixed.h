@@ -120,7 +120,7 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
 // G2P of one point from an LDS window of node velocities (or the global
 // velocity lines if the stencil left the window).
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_f *g, long long gp,
-                                          const lds_f *vwx, const lds_f *vwy, int wc0, int wr0, int win_w,
+                                          const lds_f *vw, int wc0, int wr0, int win_w,
                                           int win_h, unsigned &xq, unsigned &yq, float v[2], float C[4], float D[4],
                                           float &Jp, unsigned &ud, float &p_out, float &q_out, unsigned &e,
                                           unsigned point, unsigned step)
@@ -136,7 +136,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     weights(fx, wx);
     weights(fy, wy);
     const int lx = bxc - wc0, ly = by - wr0;
-    const bool inwin = vwx && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
+    const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
     v[0] = v[1] = 0;
     C[0] = C[1] = C[2] = C[3] = 0;
 #pragma unroll 1
@@ -146,8 +146,8 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
             float nvx, nvy;
             if (inwin) {
                 const int li = (lx + i) * win_h + ly + j;
-                nvx = vwx[li];
-                nvy = vwy[li];
+                nvx = vw[2 * li];           // (vx, vy) interleaved: one 64-bit LDS read
+                nvy = vw[2 * li + 1];
             } else {
                 const long long gi = (long long)(bxc + i) * c.gridY + by + j;
                 nvx = g[kLinePxHi * gp + gi];
