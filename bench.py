@@ -29,7 +29,13 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12          # MI355X HBM3E, bytes/s
 METRIC = "M particle-steps/s + HBM GB/s fraction, mixed vs FP64 at ≤1e-4 rel pos error"
 
-# algorithmic bytes: per point and per touched node, per launch
+# Algorithmic HBM bytes per point and per touched grid node, per launch.
+# FP64 fused G2P+P2G: reads utility, Jp_inv, x, Fe (8+8+16+32 = 64 B) and writes
+# x, v, Fe, C, Jp_inv, utility (16+16+32+32+8+8 = 112 B) per point; per node it
+# reads the velocity (16 B) and accumulates mass and momentum (24 B).  The mixed
+# form keeps 4-byte point fields (32 + 56 B); its node velocity is 8 B and the
+# compensated (hi, lo) mass/momentum pairs 24 B.  A whole substep adds the grid
+# update (per node: read 3 values, write 2, clear the other buffer).
 BYTES = {
     "fp64": {"fused_point": 64 + 112, "fused_node": 16 + 24, "p2g_point": 104, "g2p_point": 64 + 112,
              "step_point": 280, "step_node": 80},

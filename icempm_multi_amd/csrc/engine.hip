@@ -564,6 +564,7 @@ int Engine::download(int scene, double *hssoa, unsigned capacity, int offset, in
 void Engine::timed_begin(bool timed)
 {
     if (!timed) return;
+    if (ev_used_ >= 4096) collect_timings();   // bounds the event pool of long single-phase loops
     if (ev_used_ == ev_pool_.size()) {
         hipEvent_t a, b;
         hipEventCreate(&a);
@@ -621,22 +622,29 @@ int Engine::zero_grid(int scene, int buf)
 #define LAUNCH_PREC(fn, ...) \
     (precision_ == ICEMPM_PRECISION_MIXED ? mixed::fn(__VA_ARGS__) : fp64::fn(__VA_ARGS__))
 
+// The single-phase calls record hipEvents around their launch (the host-driven
+// substep loop reads per-phase times like the reference's record_timings).
 int Engine::reset_grid(int scene)
 {
     RC(ensure_committed());
-    return zero_grid(scene, 0);
+    timed_begin(true);
+    const int rc = zero_grid(scene, 0);
+    timed_end(true, kClsZero);
+    return rc;
 }
 
 int Engine::p2g(int scene)
 {
     RC(ensure_committed());
     const StepArgs a = args();
+    timed_begin(true);
     if (bins_valid_ && nscenes() == 1) {
         const int nb = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
         HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, 0, 0, 0, nb, stream_));
     } else {
         HIPCHK(LAUNCH_PREC(launch_p2g_direct, a, scene, scenes_[scene].n_pts, 0, stream_));
     }
+    timed_end(true, kClsP2G);
     scenes_[scene].clean[0] = false;
     return ICEMPM_OK;
 }
@@ -667,7 +675,9 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
     }
     a.chunks = first;
     a.n_chunks = nchunks;
+    timed_begin(true);
     HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, std::min(nchunks, n_cus_ * 8), stream_));
+    timed_end(true, kClsUpdate);
     HIPCHK(hipStreamSynchronize(stream_));   // ck lives on this stack frame
     return ICEMPM_OK;
 }
@@ -676,12 +686,14 @@ int Engine::g2p(int scene, int recordPQ)
 {
     RC(ensure_committed());
     const StepArgs a = args();
+    timed_begin(true);
     if (bins_valid_ && nscenes() == 1) {
         const int nb = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
         HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, 0, 0, recordPQ ? 1 : 0, nb, stream_));
     } else {
         HIPCHK(LAUNCH_PREC(launch_g2p_direct, a, scene, scenes_[scene].n_pts, 0, recordPQ ? 1 : 0, stream_));
     }
+    timed_end(true, kClsG2P);
     scenes_[scene].sim_step++;
     return ICEMPM_OK;
 }
