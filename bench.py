@@ -114,13 +114,25 @@ def accuracy(im, P, hssoa, n, steps):
 
 
 def batch_workload(args, im, scenes):
-    """64 independent blocks (log-uniform 10k-500k points, grids 128x128 to
-    1024x512, E ~ U[3e9, 9e9], cohesion ~ U[5e4, 2e5]), FP64 and mixed, as
-    one launch per phase, against stepping each scene with its own partition."""
+    """Many independent scenes, FP64 and mixed, stepped as one launch per phase,
+    against stepping each scene with its own partition.
+
+    batch: 64 blocks (log-uniform 10k-500k points, grids 128x128 to 1024x512,
+    E ~ U[3e9, 9e9], cohesion ~ U[5e4, 2e5]).
+    floes: 8 fields of seeded convex floes (diameters 1-20 m, power law with
+    exponent -2) on grids of 4096x1024 to 8192x2048, about 24M points."""
     t0 = time.perf_counter()
-    bag = scenes.batch_scenes(n_scenes=args.scenes)
-    n_tot = sum(n for _, _, n in bag)
-    log(f"batch: {len(bag)} scenes, {n_tot} points, setup {time.perf_counter() - t0:.1f}s")
+    if args.workload == "floes":
+        bag = scenes.floe_fields(n_scenes=args.scenes)
+        n_tot = sum(n for _, _, n in bag)
+        desc = (f"{len(bag)} floe fields, {n_tot} points, convex floes 1-20 m (power law -2), grids "
+                "4096x1024..8192x2048, h=0.05 m, dt=1e-5 s, hipGraph replay")
+    else:
+        bag = scenes.batch_scenes(n_scenes=args.scenes)
+        n_tot = sum(n for _, _, n in bag)
+        desc = (f"{len(bag)} independent ice blocks, {n_tot} points, log-uniform 10k-500k per scene, "
+                "grids 128x128..1024x512, per-scene E and cohesion, dt=1e-5 s, hipGraph replay")
+    log(f"{args.workload}: {len(bag)} scenes, {n_tot} points, setup {time.perf_counter() - t0:.1f}s")
     K = args.steps
 
     # plain: one partition per scene, scenes stepped one after the other
@@ -166,9 +178,7 @@ def batch_workload(args, im, scenes):
         "steps": K, "warmup": args.warmup, "ms_per_step": round(res["mixed"]["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{len(bag)} independent ice blocks, {n_tot} points, log-uniform 10k-500k per scene, "
-                               "grids 128x128..1024x512, per-scene E and cohesion, dt=1e-5 s, hipGraph replay",
-                   "scenes": len(bag), "points": n_tot},
+        "config": {"workload": desc, "scenes": len(bag), "points": n_tot},
         "fp64_batched": round(res["fp64"]["value"], 2),
         "fp64_plain_per_scene": round(plain, 2),
         "speedup_mixed_batched_vs_plain_fp64": round(res["mixed"]["value"] / plain, 3),
@@ -192,8 +202,8 @@ def main():
     ap.add_argument("--accuracy-steps", type=int, default=1000, help="substeps of the mixed vs FP64 comparison")
     ap.add_argument("--calib-copy", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
     ap.add_argument("--tile", default=None, help="TXxTY tile shape instead of the autotuner's choice")
-    ap.add_argument("--workload", choices=["sheet", "batch"], default="sheet")
-    ap.add_argument("--scenes", type=int, default=64, help="scenes of the batch workload")
+    ap.add_argument("--workload", choices=["sheet", "batch", "floes"], default="sheet")
+    ap.add_argument("--scenes", type=int, default=None, help="scenes of the batch (64) or floes (8) workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU stepping (x-slab halo exchange) "
@@ -205,7 +215,9 @@ def main():
 
     if args.calib_copy:
         calibration_copy()
-    if args.workload == "batch":
+    if args.workload in ("batch", "floes"):
+        if args.scenes is None:
+            args.scenes = 64 if args.workload == "batch" else 8
         return batch_workload(args, im, scenes)
 
     nx = int(round(4000 * args.scale))
@@ -227,7 +239,7 @@ def main():
             chosen = args.tile
         else:
             chosen, tune_ms = part.autotune(3)
-            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:10]]}")
+            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:15]]}")
         frame = P.UpdateEveryNthStep
         flags_timed = im.STEP_TIMINGS if not args.graph else im.STEP_GRAPH
 

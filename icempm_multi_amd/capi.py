@@ -66,6 +66,7 @@ _SIGS = {
     "icempm_partition_get_position_map": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                     C.POINTER(C.c_int)]),
     "icempm_partition_set_tile_shape": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
+    "icempm_partition_set_lane_map": (C.c_int, [_VP, C.c_int]),
     "icempm_partition_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     "icempm_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_VP)]),
     "icempm_batch_destroy": (C.c_int, [_VP]),
@@ -227,10 +228,13 @@ class Partition:
     def set_tile_shape(self, tx, ty, margin=1):
         _check(lib().icempm_partition_set_tile_shape(self._h, tx, ty, margin))
 
+    def set_lane_map(self, lane_spread):
+        _check(lib().icempm_partition_set_lane_map(self._h, lane_spread))
+
     def autotune(self, n_trials=5):
-        out = (C.c_float * 16)()
+        out = (C.c_float * 32)()
         chosen = C.c_int(-1)
-        _check(lib().icempm_partition_autotune(self._h, n_trials, out, 16, C.byref(chosen)))
+        _check(lib().icempm_partition_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
         return chosen.value, list(out)
 
     def indenter_force(self, n_subdivisions=100):
@@ -345,9 +349,9 @@ class Batch:
         return n.value, t.value
 
     def autotune(self, n_trials=3):
-        out = (C.c_float * 16)()
+        out = (C.c_float * 32)()
         chosen = C.c_int(-1)
-        _check(lib().icempm_batch_autotune(self._h, n_trials, out, 16, C.byref(chosen)))
+        _check(lib().icempm_batch_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
         return chosen.value, list(out)
 
     def last_step_ms(self):

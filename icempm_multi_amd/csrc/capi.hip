@@ -336,6 +336,16 @@ int icempm_partition_set_tile_shape(icempm_partition *p, int tx, int ty, int mar
     return guarded([&] { return p->eng->set_tile_shape(tx, ty, margin); });
 }
 
+int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread)
+{
+    if (!p) return bad("null partition");
+    if (lane_spread < 0 || lane_spread > 2) return bad("lane_spread must be 0, 1 or 2");
+    return guarded([&] {
+        p->eng->set_lane_spread(lane_spread);
+        return ICEMPM_OK;
+    });
+}
+
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen)
 {
     if (!p) return bad("null partition");

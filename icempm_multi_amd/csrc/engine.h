@@ -85,7 +85,7 @@ public:
     int tx() const { return tx_; }
     int ty() const { return ty_; }
     int set_tile_shape(int tx, int ty, int margin);
-    void set_lane_spread(int v) { lane_spread_ = v ? 1 : 0; invalidate_graphs(); }
+    void set_lane_spread(int v) { lane_spread_ = v < 0 ? 0 : (v > 2 ? 2 : v); invalidate_graphs(); }
 
 private:
     int fail(int code, const std::string &msg);

@@ -957,7 +957,8 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     if (!bins_valid_) RC(sort());
     // candidates: tile shape (tx, ty) x lane spread
     static const int cand[][3] = {{4, 32, 0}, {4, 64, 0}, {8, 16, 0}, {8, 32, 0}, {16, 16, 0},
-                                  {4, 32, 1}, {4, 64, 1}, {8, 16, 1}, {8, 32, 1}, {16, 16, 1}};
+                                  {4, 32, 1}, {4, 64, 1}, {8, 16, 1}, {8, 32, 1}, {16, 16, 1},
+                                  {4, 32, 2}, {4, 64, 2}, {8, 16, 2}, {8, 32, 2}, {16, 16, 2}};
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
     // backup: points into the sort buffer, clocks and forces on the host

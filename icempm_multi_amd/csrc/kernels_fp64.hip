@@ -100,12 +100,24 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         unsigned err = 0;
         int cidx = 0;
         // lane_spread: the lanes of one atomic instruction then hold points of
-        // different cells (4 points per cell), so their LDS targets differ
-        const int lane = tid & 63;
-        const int koff = (tid & ~63) + (a.lane_spread ? ((lane & 15) << 2) + (lane >> 4) : lane);
+        // different cells (about 4 points per cell), so their LDS targets differ.
+        //   0: lane l takes point l of its wave's 64-point group;
+        //   1: point (l%16)*4 + l/16 of the group (16 cells per group);
+        //   2: the wave takes a 128-point group in two passes, lane l taking
+        //      point (l%32)*4 + l/32, plus 2 in the second pass (32 cells).
+        static_assert(kBlock == 256, "lane spread 2 assumes 4 waves of 64");
+        const int lane = tid & 63, wave = tid >> 6;
+        const int n_it = a.lane_spread == 2 ? 2 * ((total + 2 * kBlock - 1) / (2 * kBlock))
+                                            : (total + kBlock - 1) / kBlock;
 #pragma unroll 1
-        for (int k0 = 0; k0 < total; k0 += kBlock) {
-            const int k = k0 + koff;
+        for (int it = 0; it < n_it; it++) {
+            int k;
+            if (a.lane_spread == 2)
+                k = (it >> 1) * (2 * kBlock) + wave * 128 + ((lane & 31) << 2) + (lane >> 5) + 2 * (it & 1);
+            else if (a.lane_spread == 1)
+                k = it * kBlock + (wave << 6) + ((lane & 15) << 2) + (lane >> 4);
+            else
+                k = it * kBlock + tid;
             if (k >= total) continue;
             if (k < run_pre[cidx]) cidx = 0;
             while (k >= run_pre[cidx + 1]) cidx++;

@@ -88,7 +88,7 @@ struct StepArgs {
     unsigned *err;            // per-scene error byte (max over points)
     int tx, ty, margin;       // tile shape and drift margin of the LDS windows
     int win_w, win_h;         // tx+2+2*margin, ty+2+2*margin
-    int lane_spread;          // 1: lane l of a wave takes point (l%16)*4 + l/16 of its 64-point group
+    int lane_spread;          // lane-to-point map of the tile kernels: 0 identity, 1 over 16 cells, 2 over 32 cells
 };
 
 }  // namespace icempm

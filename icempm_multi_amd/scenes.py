@@ -175,6 +175,8 @@ def floe_field(seed=0x1CE0200, grid=(8192, 2048), h=0.05, target_points=3_000_00
                     nPartitions=1)
     W = (grid[0] - 2 * halo - 16) * h
     H = (grid[1] - 24) * h
+    r_max = min(r_max, 0.4 * min(W, H))       # small grids: every floe fits inside
+    r_min = min(r_min, 0.5 * r_max)
     area_target = target_points * (h * h) / 4.0
     xs, ys, gs = [], [], []
     area = 0.0
@@ -203,3 +205,19 @@ def floe_field(seed=0x1CE0200, grid=(8192, 2048), h=0.05, target_points=3_000_00
     g = np.concatenate(gs)
     hssoa, n = load_raw_points(P, x, y, g, area)
     return P, hssoa, n
+
+
+FLOE_GRIDS = ((8192, 2048), (8192, 1024), (4096, 2048), (4096, 1024))
+
+
+def floe_fields(n_scenes=8, seed=0x1CE0500, total_points=24_000_000, grids=FLOE_GRIDS, h=0.05, dt=1e-5):
+    """Independent floe fields (the combined workload): grids cycling through
+    `grids` (up to 8192 x 2048), point counts proportional to the grid area,
+    summing to about total_points."""
+    areas = [grids[i % len(grids)][0] * grids[i % len(grids)][1] for i in range(n_scenes)]
+    out = []
+    for i in range(n_scenes):
+        g = grids[i % len(grids)]
+        target = int(total_points * areas[i] / sum(areas))
+        out.append(floe_field(seed=seed + i, grid=g, h=h, target_points=target, dt=dt))
+    return out

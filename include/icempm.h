@@ -243,6 +243,11 @@ int icempm_partition_get_position_map(icempm_partition *p, int *frac_bits, int *
  * windows widened by `margin` cells for points that drifted since the last
  * sort.  The autotuner sets it; callers may pin it. */
 int icempm_partition_set_tile_shape(icempm_partition *p, int tx, int ty, int margin);
+/* Lane-to-point map of the binned kernels (the autotuner sets it with the tile
+ * shape): 0 = lane l of a wave takes point l of its group, 1 = points of 16
+ * different cells per 16 lanes, 2 = points of 32 different cells per 32 lanes.
+ * The results do not depend on it beyond the order of floating-point sums. */
+int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread);
 
 /* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
  * the current point set and keeps the fastest.  Results (ms per launch) of each

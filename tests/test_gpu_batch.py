@@ -122,3 +122,24 @@ def test_batch_autotune_and_resort():
     assert b.error_code == 0
     n_sc, n_tot = b.counts()
     assert n_sc == len(bag) and n_tot == sum(n for _, _, n in bag)
+
+
+def test_floe_fields_batched_mixed_and_fp64():
+    """Two small floe fields (convex floes of different sizes on one grid each)
+    in one batch: FP64 against the oracle, mixed within 1e-4 in position."""
+    bag = [scenes.floe_field(seed=0x1CE0F00 + i, grid=(256, 128), target_points=12_000 + 4_000 * i)
+           for i in range(2)]
+    b, outs, _ = run_batch(bag, im.PRECISION_FP64, im.STEP_GRAPH, nsub=(50, 50))
+    bm, outs_m, _ = run_batch(bag, im.PRECISION_MIXED, im.STEP_GRAPH, nsub=(50, 50))
+    assert b.error_code == 0 and bm.error_code == 0
+    for i, (P, h, n) in enumerate(bag):
+        ref = h.copy()
+        O.sort(P, ref, n)
+        _, _, _, err = O.substeps(P, ref, n, 100, 0.0, 0)
+        assert err == 0
+        for name, line in FIELDS.items():
+            s = np.abs(ref[line, :n]).max() + 1e-300
+            d = np.abs(outs[i][line, :n] - ref[line, :n]).max()
+            assert d <= 1e-9 * s, f"floes {i} {name}: {d:.3e} vs {s:.3e}"
+        rel = np.linalg.norm(outs_m[i][4:6, :n] - ref[4:6, :n]) / np.linalg.norm(ref[4:6, :n])
+        assert rel <= 1e-4, (i, rel)

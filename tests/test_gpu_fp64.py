@@ -129,6 +129,25 @@ def test_substeps_match_oracle(flags, sort):
     assert sg == 20 and tg == t
 
 
+@pytest.mark.parametrize("lane_map", [0, 1, 2])
+@pytest.mark.parametrize("tile", [(4, 32), (8, 16), (16, 16)])
+def test_lane_maps_and_tiles_match_oracle(lane_map, tile):
+    """Every tile shape and lane-to-point map of the binned kernels, with a
+    point count that leaves ragged last groups."""
+    P, h, n = small_block(seed=0x1CE0210, nx=47, ny=13)
+    with_motion(P, h, n, seed=9)
+    ref = h.copy()
+    part = make_partition(P, h, n)
+    part.set_tile_shape(*tile)
+    part.set_lane_map(lane_map)
+    part.sort_points()
+    O.sort(P, ref, n)
+    part.step(15, im.STEP_GRAPH)
+    _, _, _, err = O.substeps(P, ref, n, 15, 0.0, 0)
+    assert err == 0 and part.error_code == 0
+    assert_close(download(part, h), ref, n, rtol=1e-9, what=f"tile={tile} lane_map={lane_map}")
+
+
 def test_frame_records_pq_and_clock():
     P, h, n = small_block(seed=0x1CE0300, dt=2e-4, E=1e8)     # UpdateEveryNthStep = 25, CFL ~0.1
     assert P.UpdateEveryNthStep == 25

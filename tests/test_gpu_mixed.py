@@ -132,6 +132,24 @@ def test_block_accuracy_vs_oracle(flags):
     assert e["crushed_mismatch"] <= max(5, 0.01 * n)
 
 
+@pytest.mark.parametrize("lane_map", [0, 1, 2])
+def test_mixed_lane_maps_match_oracle(lane_map):
+    P, h, n = scenes.block_scene(nx=47, ny=13, grid=(96, 48), n_grains=40)
+    ref = h.copy()
+    part = make(P, h, n, im.PRECISION_MIXED)
+    part.set_tile_shape(8, 16)
+    part.set_lane_map(lane_map)
+    part.sort_points()
+    O.sort(P, ref, n)
+    x0 = ref[4:6, :n].copy()
+    part.step(200, im.STEP_GRAPH)
+    O.substeps(P, ref, n, 200, 0.0, 0)
+    e = errors(download(part, h), ref, x0, n)
+    assert part.error_code == 0
+    assert e["rel_pos"] <= 1e-5
+    assert e["crushed_mismatch"] <= 2
+
+
 def test_mixed_error_code():
     P, h, n = scenes.block_scene(nx=40, ny=10, grid=(64, 32), n_grains=30)
     h[5, 3] = (P.GridY - 1.4) * P.cellsize
