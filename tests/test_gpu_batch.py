@@ -98,7 +98,7 @@ def test_batch_equals_single_partitions():
         p.transfer_from_device(o, 0)
         for line in FIELDS.values():
             s = np.abs(o[line, :n]).max() + 1e-300
-            assert np.abs(o[line, :n] - outs[i][line, :n]).max() <= 1e-11 * s
+            assert np.abs(o[line, :n] - outs[i][line, :n]).max() <= 1e-10 * s
 
 
 def test_batch_error_code_from_one_scene():

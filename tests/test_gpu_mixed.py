@@ -150,6 +150,27 @@ def test_mixed_lane_maps_match_oracle(lane_map):
     assert e["crushed_mismatch"] <= 2
 
 
+def test_block_2000_substeps_per_field_errors():
+    """The precision form of the 40 x 10 m block against the FP64 oracle after
+    2000 substeps: relative position error <= 1e-4 (the bar); the other
+    per-field errors are printed and held to loose bounds."""
+    P, h, n = scenes.block_scene(seed=0x1CE0902)
+    ref = h.copy()
+    part = make(P, h, n, im.PRECISION_MIXED)
+    part.sort_points()
+    O.sort(P, ref, n)
+    x0 = ref[4:6, :n].copy()
+    for _ in range(4):
+        part.step(500, im.STEP_GRAPH)
+    O.substeps(P, ref, n, 2000, 0.0, 0)
+    e = errors(download(part, h), ref, x0, n)
+    print("block, mixed vs FP64 oracle after 2000 substeps:", e)
+    assert part.error_code == 0
+    assert e["rel_pos"] <= 1e-4
+    assert e["rel_disp"] <= 5e-2 and e["rel_v"] <= 5e-2 and e["rel_F_minus_I"] <= 5e-2
+    assert e["crushed_mismatch"] <= max(5, 0.01 * n)
+
+
 def test_mixed_error_code():
     P, h, n = scenes.block_scene(nx=40, ny=10, grid=(64, 32), n_grains=30)
     h[5, 3] = (P.GridY - 1.4) * P.cellsize
