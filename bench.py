@@ -239,7 +239,7 @@ def main():
             chosen = args.tile
         else:
             chosen, tune_ms = part.autotune(3)
-            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:15]]}")
+            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:25]]}")
         frame = P.UpdateEveryNthStep
         flags_timed = im.STEP_TIMINGS if not args.graph else im.STEP_GRAPH
 

@@ -67,6 +67,7 @@ _SIGS = {
                                                     C.POINTER(C.c_int)]),
     "icempm_partition_set_tile_shape": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "icempm_partition_set_lane_map": (C.c_int, [_VP, C.c_int]),
+    "icempm_partition_set_tile_schedule": (C.c_int, [_VP, C.c_int]),
     "icempm_partition_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
     "icempm_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_VP)]),
     "icempm_batch_destroy": (C.c_int, [_VP]),
@@ -230,6 +231,9 @@ class Partition:
 
     def set_lane_map(self, lane_spread):
         _check(lib().icempm_partition_set_lane_map(self._h, lane_spread))
+
+    def set_tile_schedule(self, dynamic):
+        _check(lib().icempm_partition_set_tile_schedule(self._h, dynamic))
 
     def autotune(self, n_trials=5):
         out = (C.c_float * 32)()

@@ -346,6 +346,16 @@ int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread)
     });
 }
 
+int icempm_partition_set_tile_schedule(icempm_partition *p, int dynamic)
+{
+    if (!p) return bad("null partition");
+    if (dynamic != 0 && dynamic != 1) return bad("dynamic must be 0 or 1");
+    return guarded([&] {
+        p->eng->set_dyn_tiles(dynamic);
+        return ICEMPM_OK;
+    });
+}
+
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen)
 {
     if (!p) return bad("null partition");

@@ -86,6 +86,7 @@ public:
     int ty() const { return ty_; }
     int set_tile_shape(int tx, int ty, int margin);
     void set_lane_spread(int v) { lane_spread_ = v < 0 ? 0 : (v > 2 ? 2 : v); invalidate_graphs(); }
+    void set_dyn_tiles(int v) { dyn_tiles_ = v ? 1 : 0; invalidate_graphs(); }
 
 private:
     int fail(int code, const std::string &msg);
@@ -121,6 +122,7 @@ private:
     SceneDev *d_scenes_ = nullptr;
     SceneClock *d_clocks_ = nullptr;
     unsigned *d_err_ = nullptr;
+    int *d_tile_queue_ = nullptr;
     NodeChunk *d_chunks_ = nullptr;
     int n_chunks_ = 0;
 
@@ -135,6 +137,7 @@ private:
     int n_live_tiles_ = 0;
     int tx_ = 8, ty_ = 32, margin_ = 1;
     int lane_spread_ = 1;
+    int dyn_tiles_ = 0;
     std::vector<SceneDev> host_dev_;
 
     // timings

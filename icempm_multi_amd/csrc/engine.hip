@@ -51,7 +51,7 @@ Engine::~Engine()
     if (ev_a_) hipEventDestroy(ev_a_);
     if (ev_b_) hipEventDestroy(ev_b_);
     void *bufs[] = {pts_[0], pts_[1], grid_mem_, ind_force_mem_, d_scenes_, d_clocks_, d_err_, d_chunks_,
-                    bin_mem_, tile_mem_};
+                    bin_mem_, tile_mem_, d_tile_queue_};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (stream_) hipStreamDestroy(stream_);
@@ -335,6 +335,11 @@ int Engine::commit()
     HIPCHK(hipMalloc(&d_clocks_, sizeof(SceneClock) * nscenes()));
     HIPCHK(hipMalloc(&d_err_, sizeof(unsigned) * nscenes()));
     HIPCHK(hipMemsetAsync(d_err_, 0, sizeof(unsigned) * nscenes(), stream_));
+    if (!d_tile_queue_) {
+        // tile queue counters: zero here, and cleared again by the last block of every launch
+        HIPCHK(hipMalloc(&d_tile_queue_, sizeof(int) * 16));
+        HIPCHK(hipMemsetAsync(d_tile_queue_, 0, sizeof(int) * 16, stream_));
+    }
 
     std::vector<NodeChunk> chunks;
     for (int i = 0; i < nscenes(); i++) {
@@ -419,6 +424,8 @@ StepArgs Engine::args() const
     a.win_w = tx_ + 2 + 2 * margin_;
     a.win_h = ty_ + 2 + 2 * margin_;
     a.lane_spread = lane_spread_;
+    a.dyn_tiles = dyn_tiles_;
+    a.tile_queue = d_tile_queue_;
     return a;
 }
 
@@ -955,21 +962,24 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
 {
     RC(ensure_committed());
     if (!bins_valid_) RC(sort());
-    // candidates: tile shape (tx, ty) x lane spread
-    static const int cand[][3] = {{4, 32, 0}, {4, 64, 0}, {8, 16, 0}, {8, 32, 0}, {16, 16, 0},
-                                  {4, 32, 1}, {4, 64, 1}, {8, 16, 1}, {8, 32, 1}, {16, 16, 1},
-                                  {4, 32, 2}, {4, 64, 2}, {8, 16, 2}, {8, 32, 2}, {16, 16, 2}};
+    // candidates: tile shape (tx, ty) x lane spread x tile distribution (static, dynamic)
+    static const int cand[][4] = {{4, 32, 0, 0}, {4, 64, 0, 0}, {8, 16, 0, 0}, {8, 32, 0, 0}, {16, 16, 0, 0},
+                                  {4, 32, 1, 0}, {4, 64, 1, 0}, {8, 16, 1, 0}, {8, 32, 1, 0}, {16, 16, 1, 0},
+                                  {4, 32, 2, 0}, {4, 64, 2, 0}, {8, 16, 2, 0}, {8, 32, 2, 0}, {16, 16, 2, 0},
+                                  {4, 32, 1, 1}, {4, 64, 1, 1}, {8, 16, 1, 1}, {8, 32, 1, 1}, {16, 16, 1, 1},
+                                  {4, 32, 2, 1}, {4, 64, 2, 1}, {8, 16, 2, 1}, {8, 32, 2, 1}, {16, 16, 2, 1}};
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
     // backup: points into the sort buffer, clocks and forces on the host
     HIPCHK(hipMemcpyAsync(pts_[1], pts_[0], pts_bytes, hipMemcpyDeviceToDevice, stream_));
     std::vector<SceneHost> saved = scenes_;
-    const int tx0 = tx_, ty0 = ty_, ls0 = lane_spread_;
+    const int tx0 = tx_, ty0 = ty_, ls0 = lane_spread_, dy0 = dyn_tiles_;
     float best = 1e30f;
     int best_i = -1;
     n_trials = std::max(1, n_trials);
     for (int c = 0; c < ncand; c++) {
         lane_spread_ = cand[c][2];
+        dyn_tiles_ = cand[c][3];
         if (set_tile_shape(cand[c][0], cand[c][1], margin_) != ICEMPM_OK) {
             if (c < n_out && out_ms) out_ms[c] = -1;
             continue;
@@ -996,11 +1006,14 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     RC(reset_indenter_force());
     if (best_i < 0) {
         lane_spread_ = ls0;
+        dyn_tiles_ = dy0;
         RC(set_tile_shape(tx0, ty0, margin_));
     } else {
         lane_spread_ = cand[best_i][2];
+        dyn_tiles_ = cand[best_i][3];
         RC(set_tile_shape(cand[best_i][0], cand[best_i][1], margin_));
     }
+    invalidate_graphs();   // captured launches hold the lane map and distribution by value
     if (chosen) *chosen = best_i;
     HIPCHK(hipStreamSynchronize(stream_));
     return ICEMPM_OK;

@@ -47,7 +47,39 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     const int ntiles = *a.n_tiles;
     const PtRef R{(double *)a.pts, a.pitch};
 
-    for (int w = xcd_block(); w < ntiles; w += gridDim.x) {
+    // Tile distribution.  Static: block b takes tiles b', b'+G, ... (b' the
+    // XCD-contiguous remap of b).  Dynamic: each XCD (blockIdx.x % 8, the
+    // hardware's round-robin dispatch order) owns a contiguous eighth of the
+    // tile list; its blocks pull tiles with an atomic counter and then help the
+    // other XCDs' ranges.  Every block counts itself done once at the end; the
+    // last one clears the counters for the next launch.
+    __shared__ int s_tile;
+    const bool dyn = a.dyn_tiles != 0;
+    const int nq = (gridDim.x >= 8 && (gridDim.x & 7) == 0) ? 8 : 1;
+    const int q0 = nq == 8 ? (int)(blockIdx.x & 7) : 0;
+    int qk = 0;                      // queues visited so far (dynamic)
+    int w_static = xcd_block();
+    for (;;) {
+        int w;
+        if (dyn) {
+            if (tid == 0) {
+                int t = -1;
+                for (; qk < nq; qk++) {
+                    const int q = (q0 + qk) % nq;
+                    const int hi = (int)((long long)ntiles * (q + 1) / nq);
+                    const int got = (int)((long long)ntiles * q / nq) + atomicAdd(a.tile_queue + q, 1);
+                    if (got < hi) { t = got; break; }
+                }
+                s_tile = t;
+            }
+            __syncthreads();
+            w = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps tile data in scalar registers
+            if (w < 0) break;
+        } else {
+            w = w_static;
+            w_static += gridDim.x;
+            if (w >= ntiles) break;
+        }
         const TileItem ti = a.tiles[w];
         const SceneDev &S = a.scenes[ti.scene];
         const SceneConsts &c = S.c;
@@ -109,8 +141,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         const int lane = tid & 63, wave = tid >> 6;
         const int n_it = a.lane_spread == 2 ? 2 * ((total + 2 * kBlock - 1) / (2 * kBlock))
                                             : (total + kBlock - 1) / kBlock;
-#pragma unroll 1
-        for (int it = 0; it < n_it; it++) {
+        // point k of the tile -> SoA index; false past the end
+        auto locate = [&](int it, long long &pt) -> bool {
             int k;
             if (a.lane_spread == 2)
                 k = (it >> 1) * (2 * kBlock) + wave * 128 + ((lane & 31) << 2) + (lane >> 5) + 2 * (it & 1);
@@ -118,14 +150,34 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                 k = it * kBlock + (wave << 6) + ((lane & 15) << 2) + (lane >> 4);
             else
                 k = it * kBlock + tid;
-            if (k >= total) continue;
+            if (k >= total) return false;
             if (k < run_pre[cidx]) cidx = 0;
             while (k >= run_pre[cidx + 1]) cidx++;
-            const long long pt = run_start[cidx] + (k - run_pre[cidx]);
-            long long ud = as_ll(R.at(kLineUtility, pt));
+            pt = run_start[cidx] + (k - run_pre[cidx]);
+            return true;
+        };
+        // the state of the next point is loaded while the current one is computed
+        long long pt_n = 0, ud_n = 0;
+        double x_n = 0, y_n = 0, F_n[4] = {0, 0, 0, 0}, Jp_n = 0;
+        bool ok_n = n_it > 0 && locate(0, pt_n);
+        if (ok_n) {
+            ud_n = as_ll(R.at(kLineUtility, pt_n));
+            load_state(R, pt_n, x_n, y_n, F_n, Jp_n);
+        }
+#pragma unroll 1
+        for (int it = 0; it < n_it; it++) {
+            const bool ok = ok_n;
+            const long long pt = pt_n;
+            long long ud = ud_n;
+            double x = x_n, y = y_n, F[4] = {F_n[0], F_n[1], F_n[2], F_n[3]}, Jp = Jp_n;
+            ok_n = it + 1 < n_it && locate(it + 1, pt_n);
+            if (ok_n) {
+                ud_n = as_ll(R.at(kLineUtility, pt_n));
+                load_state(R, pt_n, x_n, y_n, F_n, Jp_n);
+            }
+            if (!ok) continue;
             if (ud & kDisabled) continue;
-            double x, y, F[4], Jp, v[2], B[4];
-            load_state(R, pt, x, y, F, Jp);
+            double v[2], B[4];
             if (DO_G2P) {
                 double pp, qq;
                 unsigned e;
@@ -175,6 +227,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         }
         __syncthreads();
     }
+    if (dyn && tid == 0) {
+        if (atomicAdd(a.tile_queue + 8, 1) == (int)gridDim.x - 1)
+            for (int k = 0; k < 9; k++) a.tile_queue[k] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -183,6 +239,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
 {
+    // kU nodes per thread with their loads in flight together: the masses of
+    // both buffers first, then the momenta of the nodes that hold mass
+    constexpr int kU = 4;
     for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
         const NodeChunk nc = a.chunks[ch];
         const SceneDev &S = a.scenes[nc.scene];
@@ -192,53 +251,36 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         double *go = (double *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
-        const double dt = c.dt;
-        for (int k = threadIdx.x; k < nc.count; k += kBlock) {
-            const long long idx = (long long)nc.node0 + k;
-            if (zero_other && go[idx] != 0) {
-                go[idx] = 0;
-                go[gp + idx] = 0;
-                go[2 * gp + idx] = 0;
+        for (int k0 = threadIdx.x; k0 < nc.count; k0 += kU * kBlock) {
+            double m[kU], mo[kU], px[kU], py[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int k = k0 + u * kBlock;
+                const long long idx = (long long)nc.node0 + k;
+                m[u] = k < nc.count ? g[idx] : 0.0;
+                mo[u] = (zero_other && k < nc.count) ? go[idx] : 0.0;
             }
-            const double mass = g[idx];
-            if (mass == 0) continue;
-            const double px = g[gp + idx], py = g[2 * gp + idx];
-            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
-            const double gnx = (double)gix * c.h, gny = (double)giy * c.h;
-            double vx = px / mass, vy = py / mass;
-            vy -= c.dt_gravity;
-            const double sq = vx * vx + vy * vy;
-            if (sq > c.vmax_sq) {
-                const double r = sqrt(sq);
-                vx = vx / r * c.vmax;
-                vy = vy / r * c.vmax;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
+                px[u] = m[u] != 0 ? g[gp + idx] : 0.0;
+                py[u] = m[u] != 0 ? g[2 * gp + idx] : 0.0;
             }
-            double n0 = gnx - clk.ind_x, n1 = gny - clk.ind_y;
-            if (n0 * n0 + n1 * n1 < c.ind_rsq) {
-                const double vr0 = vx - c.ind_vel, vr1 = vy;
-                const double z = n0 * n0 + n1 * n1;
-                if (z > 0) { const double r = sqrt(z); n0 = n0 / r; n1 = n1 / r; }
-                const double vn = vr0 * n0 + vr1 * n1;
-                if (vn < 0) {
-                    const double vt0 = vr0 - n0 * vn, vt1 = vr1 - n1 * vn;
-                    const double pv0 = vx, pv1 = vy;
-                    vx = c.ind_vel + vt0;
-                    vy = 0 + vt1;
-                    const double f0 = (pv0 - vx) * mass / dt, f1 = (pv1 - vy) * mass / dt;
-                    float angle = atan2f((float)n0, (float)n1);
-                    angle += kPi;
-                    angle *= c.n_ind_sub / (2 * kPi);
-                    const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
-                    atomicAdd(&force[2 * index], f0);
-                    atomicAdd(&force[2 * index + 1], f1);
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
+                if (mo[u] != 0) {
+                    go[idx] = 0;
+                    go[gp + idx] = 0;
+                    go[2 * gp + idx] = 0;
                 }
+                if (m[u] == 0) continue;
+                const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+                double vx, vy;
+                node_velocity(c, clk, force, gix, giy, m[u], px[u], py[u], vx, vy);
+                g[gp + idx] = vx;
+                g[2 * gp + idx] = vy;
             }
-            if (giy <= 2) { vx = 0; vy = 0; }
-            if (giy >= c.gridY - 3 && vy > 0) vy = 0;
-            if (gix <= 2 && vx < 0) vx = 0;
-            else if (gix >= c.gridXTotal - 3 && vx > 0) vx = 0;
-            g[gp + idx] = vx;
-            g[2 * gp + idx] = vy;
         }
     }
 }

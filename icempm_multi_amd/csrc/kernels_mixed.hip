@@ -34,7 +34,39 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
     const int ntiles = *a.n_tiles;
     const PtRef R{(unsigned *)a.pts, a.pitch};
 
-    for (int w = xcd_block(); w < ntiles; w += gridDim.x) {
+    // Tile distribution.  Static: block b takes tiles b', b'+G, ... (b' the
+    // XCD-contiguous remap of b).  Dynamic: each XCD (blockIdx.x % 8, the
+    // hardware's round-robin dispatch order) owns a contiguous eighth of the
+    // tile list; its blocks pull tiles with an atomic counter and then help the
+    // other XCDs' ranges.  Every block counts itself done once at the end; the
+    // last one clears the counters for the next launch.
+    __shared__ int s_tile;
+    const bool dyn = a.dyn_tiles != 0;
+    const int nq = (gridDim.x >= 8 && (gridDim.x & 7) == 0) ? 8 : 1;
+    const int q0 = nq == 8 ? (int)(blockIdx.x & 7) : 0;
+    int qk = 0;                      // queues visited so far (dynamic)
+    int w_static = xcd_block();
+    for (;;) {
+        int w;
+        if (dyn) {
+            if (tid == 0) {
+                int t = -1;
+                for (; qk < nq; qk++) {
+                    const int q = (q0 + qk) % nq;
+                    const int hi = (int)((long long)ntiles * (q + 1) / nq);
+                    const int got = (int)((long long)ntiles * q / nq) + atomicAdd(a.tile_queue + q, 1);
+                    if (got < hi) { t = got; break; }
+                }
+                s_tile = t;
+            }
+            __syncthreads();
+            w = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps tile data in scalar registers
+            if (w < 0) break;
+        } else {
+            w = w_static;
+            w_static += gridDim.x;
+            if (w >= ntiles) break;
+        }
         const TileItem ti = a.tiles[w];
         const SceneDev &S = a.scenes[ti.scene];
         const SceneConsts &c = S.c;
@@ -93,8 +125,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         const int lane = tid & 63, wave = tid >> 6;
         const int n_it = a.lane_spread == 2 ? 2 * ((total + 2 * kBlock - 1) / (2 * kBlock))
                                             : (total + kBlock - 1) / kBlock;
-#pragma unroll 1
-        for (int it = 0; it < n_it; it++) {
+        // point k of the tile -> SoA index; false past the end
+        auto locate = [&](int it, long long &pt) -> bool {
             int k;
             if (a.lane_spread == 2)
                 k = (it >> 1) * (2 * kBlock) + wave * 128 + ((lane & 31) << 2) + (lane >> 5) + 2 * (it & 1);
@@ -102,15 +134,34 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 k = it * kBlock + (wave << 6) + ((lane & 15) << 2) + (lane >> 4);
             else
                 k = it * kBlock + tid;
-            if (k >= total) continue;
+            if (k >= total) return false;
             if (k < run_pre[cidx]) cidx = 0;
             while (k >= run_pre[cidx + 1]) cidx++;
-            const long long pt = run_start[cidx] + (k - run_pre[cidx]);
-            unsigned ud = R.u(kLineUtility, pt);
+            pt = run_start[cidx] + (k - run_pre[cidx]);
+            return true;
+        };
+        // the state of the next point is loaded while the current one is computed
+        long long pt_n = 0;
+        unsigned ud_n = 0, xq_n = 0, yq_n = 0;
+        float D_n[4] = {0, 0, 0, 0}, Jp_n = 0;
+        bool ok_n = n_it > 0 && locate(0, pt_n);
+        if (ok_n) {
+            ud_n = R.u(kLineUtility, pt_n);
+            load_state(R, pt_n, xq_n, yq_n, D_n, Jp_n);
+        }
+#pragma unroll 1
+        for (int it = 0; it < n_it; it++) {
+            const bool ok = ok_n;
+            const long long pt = pt_n;
+            unsigned ud = ud_n, xq = xq_n, yq = yq_n;
+            float D[4] = {D_n[0], D_n[1], D_n[2], D_n[3]}, Jp = Jp_n, v[2], C[4];
+            ok_n = it + 1 < n_it && locate(it + 1, pt_n);
+            if (ok_n) {
+                ud_n = R.u(kLineUtility, pt_n);
+                load_state(R, pt_n, xq_n, yq_n, D_n, Jp_n);
+            }
+            if (!ok) continue;
             if (ud & (unsigned)kDisabled) continue;
-            unsigned xq, yq;
-            float D[4], Jp, v[2], C[4];
-            load_state(R, pt, xq, yq, D, Jp);
             if (DO_G2P) {
                 float pp, qq;
                 unsigned e;
@@ -161,11 +212,17 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         }
         __syncthreads();
     }
+    if (dyn && tid == 0) {
+        if (atomicAdd(a.tile_queue + 8, 1) == (int)gridDim.x - 1)
+            for (int k = 0; k < 9; k++) a.tile_queue[k] = 0;
+    }
 }
 
 // Grid update (kernels.cuh:164-301) in FP64 on hi + lo; velocity stored in FP32.
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
 {
+    // kU nodes per thread with their loads in flight together (see the FP64 form)
+    constexpr int kU = 4;
     for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
         const NodeChunk nc = a.chunks[ch];
         const SceneDev &S = a.scenes[nc.scene];
@@ -175,53 +232,43 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         glb_f *go = (glb_f *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
-        const double dt = c.dt;
-        for (int k = threadIdx.x; k < nc.count; k += kBlock) {
-            const long long idx = (long long)nc.node0 + k;
-            if (zero_other && go[idx] != 0) {
+        for (int k0 = threadIdx.x; k0 < nc.count; k0 += kU * kBlock) {
+            float mh[kU], mo[kU];
 #pragma unroll
-                for (int l = 0; l < 6; l++) go[l * gp + idx] = 0;
+            for (int u = 0; u < kU; u++) {
+                const int k = k0 + u * kBlock;
+                const long long idx = (long long)nc.node0 + k;
+                mh[u] = k < nc.count ? g[idx] : 0.f;
+                mo[u] = (zero_other && k < nc.count) ? go[idx] : 0.f;
             }
-            const float mh = g[idx];
-            if (mh == 0) continue;
-            const double mass = (double)mh + (double)g[kLineMLo * gp + idx];
-            const double px = (double)g[kLinePxHi * gp + idx] + (double)g[kLinePxLo * gp + idx];
-            const double py = (double)g[kLinePyHi * gp + idx] + (double)g[kLinePyLo * gp + idx];
-            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
-            const double gnx = (double)gix * c.h, gny = (double)giy * c.h;
-            double vx = px / mass, vy = py / mass;
-            vy -= c.dt_gravity;
-            const double sq = vx * vx + vy * vy;
-            if (sq > c.vmax_sq) {
-                const double r = sqrt(sq);
-                vx = vx / r * c.vmax;
-                vy = vy / r * c.vmax;
+            float ml[kU], pxh[kU], pxl[kU], pyh[kU], pyl[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
+                const bool live = mh[u] != 0;
+                ml[u] = live ? g[kLineMLo * gp + idx] : 0.f;
+                pxh[u] = live ? g[kLinePxHi * gp + idx] : 0.f;
+                pxl[u] = live ? g[kLinePxLo * gp + idx] : 0.f;
+                pyh[u] = live ? g[kLinePyHi * gp + idx] : 0.f;
+                pyl[u] = live ? g[kLinePyLo * gp + idx] : 0.f;
             }
-            double n0 = gnx - clk.ind_x, n1 = gny - clk.ind_y;
-            if (n0 * n0 + n1 * n1 < c.ind_rsq) {
-                const double vr0 = vx - c.ind_vel, vr1 = vy;
-                const double z = n0 * n0 + n1 * n1;
-                if (z > 0) { const double r = sqrt(z); n0 = n0 / r; n1 = n1 / r; }
-                const double vn = vr0 * n0 + vr1 * n1;
-                if (vn < 0) {
-                    const double pv0 = vx, pv1 = vy;
-                    vx = c.ind_vel + (vr0 - n0 * vn);
-                    vy = vr1 - n1 * vn;
-                    const double f0 = (pv0 - vx) * mass / dt, f1 = (pv1 - vy) * mass / dt;
-                    float angle = atan2f((float)n0, (float)n1);
-                    angle += fp64::kPi;
-                    angle *= c.n_ind_sub / (2 * fp64::kPi);
-                    const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
-                    atomicAdd(&force[2 * index], f0);
-                    atomicAdd(&force[2 * index + 1], f1);
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
+                if (mo[u] != 0) {
+#pragma unroll
+                    for (int l = 0; l < 6; l++) go[l * gp + idx] = 0;
                 }
+                if (mh[u] == 0) continue;
+                const double mass = (double)mh[u] + (double)ml[u];
+                const double px = (double)pxh[u] + (double)pxl[u];
+                const double py = (double)pyh[u] + (double)pyl[u];
+                const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+                double vx, vy;
+                fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
+                g[kLinePxHi * gp + idx] = (float)vx;
+                g[kLinePyHi * gp + idx] = (float)vy;
             }
-            if (giy <= 2) { vx = 0; vy = 0; }
-            if (giy >= c.gridY - 3 && vy > 0) vy = 0;
-            if (gix <= 2 && vx < 0) vx = 0;
-            else if (gix >= c.gridXTotal - 3 && vx > 0) vx = 0;
-            g[kLinePxHi * gp + idx] = (float)vx;
-            g[kLinePyHi * gp + idx] = (float)vy;
         }
     }
 }

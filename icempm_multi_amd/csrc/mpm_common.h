@@ -89,6 +89,8 @@ struct StepArgs {
     int tx, ty, margin;       // tile shape and drift margin of the LDS windows
     int win_w, win_h;         // tx+2+2*margin, ty+2+2*margin
     int lane_spread;          // lane-to-point map of the tile kernels: 0 identity, 1 over 16 cells, 2 over 32 cells
+    int dyn_tiles;            // 1: tiles pulled from per-XCD atomic queues, 0: static round robin
+    int *tile_queue;          // 8 queue counters + 1 finished-block counter, zero between launches
 };
 
 }  // namespace icempm

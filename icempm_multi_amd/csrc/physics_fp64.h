@@ -86,6 +86,50 @@ __device__ __forceinline__ unsigned stencil_error(const SceneConsts &c, int bxc,
     return e;
 }
 
+// Velocity of one grid node with mass (kernels.cuh:178-300, live lines): p/m,
+// gravity, the vmax clamp, the moving indenter (its reaction force binned by
+// angle into `force`), and the walls.  gix, giy are global node indices.
+__device__ __forceinline__ void node_velocity(const SceneConsts &c, const SceneClock &clk, double *force, int gix,
+                                              int giy, double mass, double px, double py, double &vx_out,
+                                              double &vy_out)
+{
+    const double gnx = (double)gix * c.h, gny = (double)giy * c.h;
+    double vx = px / mass, vy = py / mass;
+    vy -= c.dt_gravity;
+    const double sq = vx * vx + vy * vy;
+    if (sq > c.vmax_sq) {
+        const double r = sqrt(sq);
+        vx = vx / r * c.vmax;
+        vy = vy / r * c.vmax;
+    }
+    double n0 = gnx - clk.ind_x, n1 = gny - clk.ind_y;
+    if (n0 * n0 + n1 * n1 < c.ind_rsq) {
+        const double vr0 = vx - c.ind_vel, vr1 = vy;
+        const double z = n0 * n0 + n1 * n1;
+        if (z > 0) { const double r = sqrt(z); n0 = n0 / r; n1 = n1 / r; }
+        const double vn = vr0 * n0 + vr1 * n1;
+        if (vn < 0) {
+            const double vt0 = vr0 - n0 * vn, vt1 = vr1 - n1 * vn;
+            const double pv0 = vx, pv1 = vy;
+            vx = c.ind_vel + vt0;
+            vy = 0 + vt1;
+            const double f0 = (pv0 - vx) * mass / c.dt, f1 = (pv1 - vy) * mass / c.dt;
+            float angle = atan2f((float)n0, (float)n1);
+            angle += kPi;
+            angle *= c.n_ind_sub / (2 * kPi);
+            const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
+            atomicAdd(&force[2 * index], f0);
+            atomicAdd(&force[2 * index + 1], f1);
+        }
+    }
+    if (giy <= 2) { vx = 0; vy = 0; }
+    if (giy >= c.gridY - 3 && vy > 0) vy = 0;
+    if (gix <= 2 && vx < 0) vx = 0;
+    else if (gix >= c.gridXTotal - 3 && vx > 0) vx = 0;
+    vx_out = vx;
+    vy_out = vy;
+}
+
 // ---- constitutive update of partition_kernel_g2p (kernels.cuh:383-390) ----
 
 struct Givens { double c, s; };

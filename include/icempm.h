@@ -248,6 +248,9 @@ int icempm_partition_set_tile_shape(icempm_partition *p, int tx, int ty, int mar
  * different cells per 16 lanes, 2 = points of 32 different cells per 32 lanes.
  * The results do not depend on it beyond the order of floating-point sums. */
 int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread);
+/* Tile distribution of the binned kernels: 0 = static round robin over the
+ * tile list, 1 = per-XCD atomic tile queues with stealing.  Set by the autotuner. */
+int icempm_partition_set_tile_schedule(icempm_partition *p, int dynamic);
 
 /* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
  * the current point set and keeps the fastest.  Results (ms per launch) of each

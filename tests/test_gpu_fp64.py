@@ -129,17 +129,19 @@ def test_substeps_match_oracle(flags, sort):
     assert sg == 20 and tg == t
 
 
+@pytest.mark.parametrize("dynamic", [0, 1])
 @pytest.mark.parametrize("lane_map", [0, 1, 2])
 @pytest.mark.parametrize("tile", [(4, 32), (8, 16), (16, 16)])
-def test_lane_maps_and_tiles_match_oracle(lane_map, tile):
-    """Every tile shape and lane-to-point map of the binned kernels, with a
-    point count that leaves ragged last groups."""
+def test_lane_maps_and_tiles_match_oracle(lane_map, tile, dynamic):
+    """Every tile shape, lane-to-point map and tile distribution of the binned
+    kernels, with a point count that leaves ragged last groups."""
     P, h, n = small_block(seed=0x1CE0210, nx=47, ny=13)
     with_motion(P, h, n, seed=9)
     ref = h.copy()
     part = make_partition(P, h, n)
     part.set_tile_shape(*tile)
     part.set_lane_map(lane_map)
+    part.set_tile_schedule(dynamic)
     part.sort_points()
     O.sort(P, ref, n)
     part.step(15, im.STEP_GRAPH)

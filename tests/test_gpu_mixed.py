@@ -132,13 +132,14 @@ def test_block_accuracy_vs_oracle(flags):
     assert e["crushed_mismatch"] <= max(5, 0.01 * n)
 
 
-@pytest.mark.parametrize("lane_map", [0, 1, 2])
-def test_mixed_lane_maps_match_oracle(lane_map):
+@pytest.mark.parametrize("lane_map,dynamic", [(0, 0), (1, 0), (2, 0), (1, 1), (2, 1)])
+def test_mixed_lane_maps_match_oracle(lane_map, dynamic):
     P, h, n = scenes.block_scene(nx=47, ny=13, grid=(96, 48), n_grains=40)
     ref = h.copy()
     part = make(P, h, n, im.PRECISION_MIXED)
     part.set_tile_shape(8, 16)
     part.set_lane_map(lane_map)
+    part.set_tile_schedule(dynamic)
     part.sort_points()
     O.sort(P, ref, n)
     x0 = ref[4:6, :n].copy()
