@@ -18,7 +18,6 @@
 
 namespace icempm {
 
-constexpr int kUpdateChunk = 1024;  // nodes per grid-update work item (4 per thread, persistent workgroups)
 
 enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kNumCls };
 
@@ -30,6 +29,7 @@ struct SceneHost {
     long long pt_off = 0;
     long long cells_off = 0;
     long long grid_elems = 0; // gpitch
+    long long seg_off = 0;    // first segment flag of the scene (per buffer)
     void *grid[2] = {nullptr, nullptr};
     double *ind_force = nullptr;
     bool clean[2] = {true, true};
@@ -123,6 +123,8 @@ private:
     SceneClock *d_clocks_ = nullptr;
     unsigned *d_err_ = nullptr;
     int *d_tile_queue_ = nullptr;
+    int *d_dirty_ = nullptr;   // 2 * n_segs_ segment flags
+    long long n_segs_ = 0;
     NodeChunk *d_chunks_ = nullptr;
     int n_chunks_ = 0;
 

@@ -51,7 +51,7 @@ Engine::~Engine()
     if (ev_a_) hipEventDestroy(ev_a_);
     if (ev_b_) hipEventDestroy(ev_b_);
     void *bufs[] = {pts_[0], pts_[1], grid_mem_, ind_force_mem_, d_scenes_, d_clocks_, d_err_, d_chunks_,
-                    bin_mem_, tile_mem_, d_tile_queue_};
+                    bin_mem_, tile_mem_, d_tile_queue_, d_dirty_};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (stream_) hipStreamDestroy(stream_);
@@ -216,6 +216,7 @@ int Engine::build_scene_dev(int i, SceneDev &d) const
     d.n_tiles_x = (c.ncols + tx_ - 1) / tx_;
     d.n_tiles_y = (c.gridY + ty_ - 1) / ty_;
     d.force_off = s.ind_force - ind_force_mem_;
+    d.seg_off = s.seg_off;
     return ICEMPM_OK;
 }
 
@@ -316,6 +317,15 @@ int Engine::commit()
     const size_t buf_bytes = (size_t)gelems * eb;
     HIPCHK(hipMalloc(&grid_mem_, 2 * buf_bytes));
     HIPCHK(hipMemsetAsync(grid_mem_, 0, 2 * buf_bytes, stream_));
+    // segment flags (grid_elems is a multiple of the segment size)
+    n_segs_ = 0;
+    for (int i = 0; i < nscenes(); i++) {
+        scenes_[i].seg_off = n_segs_;
+        n_segs_ += scenes_[i].grid_elems >> kSegShift;
+    }
+    if (d_dirty_) { hipFree(d_dirty_); d_dirty_ = nullptr; }
+    HIPCHK(hipMalloc(&d_dirty_, sizeof(int) * 2 * std::max<long long>(n_segs_, 1)));
+    HIPCHK(hipMemsetAsync(d_dirty_, 0, sizeof(int) * 2 * std::max<long long>(n_segs_, 1), stream_));
     if (ind_force_mem_) hipFree(ind_force_mem_);
     HIPCHK(hipMalloc(&ind_force_mem_, sizeof(double) * std::max<long long>(forces, 1)));
     HIPCHK(hipMemsetAsync(ind_force_mem_, 0, sizeof(double) * std::max<long long>(forces, 1), stream_));
@@ -426,6 +436,8 @@ StepArgs Engine::args() const
     a.lane_spread = lane_spread_;
     a.dyn_tiles = dyn_tiles_;
     a.tile_queue = d_tile_queue_;
+    a.dirty = d_dirty_;
+    a.dirty_stride = n_segs_;
     return a;
 }
 
@@ -622,6 +634,8 @@ int Engine::zero_grid(int scene, int buf)
     const SceneHost &s = scenes_[scene];
     HIPCHK(launch_fill_zero(s.grid[buf], (size_t)s.grid_elems * (precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3) *
                                              elem_bytes(), stream_));
+    HIPCHK(launch_fill_zero(d_dirty_ + buf * n_segs_ + s.seg_off, sizeof(int) * (s.grid_elems >> kSegShift),
+                            stream_));
     scenes_[scene].clean[buf] = true;
     return ICEMPM_OK;
 }
@@ -856,6 +870,7 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
                                                        (precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3) * elem_bytes(),
                                         stream_));
             }
+            HIPCHK(launch_fill_zero(d_dirty_, sizeof(int) * n_segs_, stream_));
             timed_end(timed, kClsZero);
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++)

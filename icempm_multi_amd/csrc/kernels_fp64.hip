@@ -206,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                             glb_add(gw + gi, dm);
                             glb_add(gw + gp + gi, dpx);
                             glb_add(gw + 2 * gp + gi, dpy);
+                            mark_dirty(a, S, wbuf, gi);
                         }
                     });
                 err = max(err, e);
@@ -223,6 +224,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                 glb_add(gw + gi, m);
                 glb_add(gw + gp + gi, apx[i]);
                 glb_add(gw + 2 * gp + gi, apy[i]);
+                mark_dirty(a, S, wbuf, gi);
             }
         }
         __syncthreads();
@@ -239,9 +241,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
 {
-    // kU nodes per thread with their loads in flight together: the masses of
-    // both buffers first, then the momenta of the nodes that hold mass
-    constexpr int kU = 4;
+    // Each wave takes 4 of the chunk's 16 segments of 64 nodes, one node per
+    // lane.  A segment of this buffer is read only if flagged; a flagged
+    // segment of the other buffer is cleared (all 64 nodes) for the next P2G
+    // and its flag reset.  The loads of the 4 segments are in flight together.
+    constexpr int kU = kUpdateChunk / kSegNodes / (kBlock / 64);   // 4
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
         const NodeChunk nc = a.chunks[ch];
         const SceneDev &S = a.scenes[nc.scene];
@@ -251,36 +256,48 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         double *go = (double *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
-        for (int k0 = threadIdx.x; k0 < nc.count; k0 += kU * kBlock) {
-            double m[kU], mo[kU], px[kU], py[kU];
+        int *fl = a.dirty + buf * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
+        int *flo = a.dirty + (buf ^ 1) * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
+        int f[kU], fo[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const int k = k0 + u * kBlock;
-                const long long idx = (long long)nc.node0 + k;
-                m[u] = k < nc.count ? g[idx] : 0.0;
-                mo[u] = (zero_other && k < nc.count) ? go[idx] : 0.0;
-            }
+        for (int u = 0; u < kU; u++) {
+            const int sl = wave + u * (kBlock / 64);
+            const bool in = sl * kSegNodes < nc.count;
+            f[u] = in ? fl[sl] : 0;
+            fo[u] = (in && zero_other) ? flo[sl] : 0;
+        }
+        double m[kU], px[kU], py[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
-                px[u] = m[u] != 0 ? g[gp + idx] : 0.0;
-                py[u] = m[u] != 0 ? g[2 * gp + idx] : 0.0;
-            }
+        for (int u = 0; u < kU; u++) {
+            const int k = (wave + u * (kBlock / 64)) * kSegNodes + lane;
+            const long long idx = (long long)nc.node0 + k;
+            m[u] = (f[u] && k < nc.count) ? g[idx] : 0.0;
+        }
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
-                if (mo[u] != 0) {
+        for (int u = 0; u < kU; u++) {
+            const long long idx = (long long)nc.node0 + (wave + u * (kBlock / 64)) * kSegNodes + lane;
+            px[u] = m[u] != 0 ? g[gp + idx] : 0.0;
+            py[u] = m[u] != 0 ? g[2 * gp + idx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int sl = wave + u * (kBlock / 64);
+            const int k = sl * kSegNodes + lane;
+            const long long idx = (long long)nc.node0 + k;
+            if (fo[u]) {
+                if (k < nc.count) {
                     go[idx] = 0;
                     go[gp + idx] = 0;
                     go[2 * gp + idx] = 0;
                 }
-                if (m[u] == 0) continue;
-                const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
-                double vx, vy;
-                node_velocity(c, clk, force, gix, giy, m[u], px[u], py[u], vx, vy);
-                g[gp + idx] = vx;
-                g[2 * gp + idx] = vy;
+                if (lane == 0) flo[sl] = 0;
             }
+            if (m[u] == 0) continue;
+            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+            double vx, vy;
+            node_velocity(c, clk, force, gix, giy, m[u], px[u], py[u], vx, vy);
+            g[gp + idx] = vx;
+            g[2 * gp + idx] = vy;
         }
     }
 }

@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
         glb_add(gw + gi, dm);
         glb_add(gw + gp + gi, dpx);
         glb_add(gw + 2 * gp + gi, dpy);
+        mark_dirty(a, S, wbuf, gi);
     });
     if (e) atomicMax(&a.err[scene], e);
 }

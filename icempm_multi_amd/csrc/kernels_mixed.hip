@@ -17,7 +17,8 @@ namespace icempm {
 namespace mixed {
 
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+// 4 waves per SIMD (<= 128 VGPRs): a few spills, but more latency hidden than at 3
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 {
     constexpr bool DO_G2P = (MODE & kTileG2P) != 0;
@@ -191,6 +192,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                             twosum_add(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm, 0.f);
                             twosum_add(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx, 0.f);
                             twosum_add(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy, 0.f);
+                            mark_dirty(a, S, wbuf, gi);
                         }
                     });
                 err = max(err, e);
@@ -208,6 +210,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, m);
                 twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, apx[i]);
                 twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, apy[i]);
+                mark_dirty(a, S, wbuf, gi);
             }
         }
         __syncthreads();
@@ -221,8 +224,9 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 // Grid update (kernels.cuh:164-301) in FP64 on hi + lo; velocity stored in FP32.
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
 {
-    // kU nodes per thread with their loads in flight together (see the FP64 form)
-    constexpr int kU = 4;
+    // segments of 64 nodes as in the FP64 form; 6 FP32 lines per buffer
+    constexpr int kU = kUpdateChunk / kSegNodes / (kBlock / 64);   // 4
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
         const NodeChunk nc = a.chunks[ch];
         const SceneDev &S = a.scenes[nc.scene];
@@ -232,43 +236,55 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         glb_f *go = (glb_f *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
-        for (int k0 = threadIdx.x; k0 < nc.count; k0 += kU * kBlock) {
-            float mh[kU], mo[kU];
+        int *fl = a.dirty + buf * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
+        int *flo = a.dirty + (buf ^ 1) * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
+        int f[kU], fo[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const int k = k0 + u * kBlock;
-                const long long idx = (long long)nc.node0 + k;
-                mh[u] = k < nc.count ? g[idx] : 0.f;
-                mo[u] = (zero_other && k < nc.count) ? go[idx] : 0.f;
-            }
-            float ml[kU], pxh[kU], pxl[kU], pyh[kU], pyl[kU];
+        for (int u = 0; u < kU; u++) {
+            const int sl = wave + u * (kBlock / 64);
+            const bool in = sl * kSegNodes < nc.count;
+            f[u] = in ? fl[sl] : 0;
+            fo[u] = (in && zero_other) ? flo[sl] : 0;
+        }
+        float mh[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
-                const bool live = mh[u] != 0;
-                ml[u] = live ? g[kLineMLo * gp + idx] : 0.f;
-                pxh[u] = live ? g[kLinePxHi * gp + idx] : 0.f;
-                pxl[u] = live ? g[kLinePxLo * gp + idx] : 0.f;
-                pyh[u] = live ? g[kLinePyHi * gp + idx] : 0.f;
-                pyl[u] = live ? g[kLinePyLo * gp + idx] : 0.f;
-            }
+        for (int u = 0; u < kU; u++) {
+            const int k = (wave + u * (kBlock / 64)) * kSegNodes + lane;
+            const long long idx = (long long)nc.node0 + k;
+            mh[u] = (f[u] && k < nc.count) ? g[idx] : 0.f;
+        }
+        float ml[kU], pxh[kU], pxl[kU], pyh[kU], pyl[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const long long idx = (long long)nc.node0 + k0 + u * kBlock;
-                if (mo[u] != 0) {
+        for (int u = 0; u < kU; u++) {
+            const long long idx = (long long)nc.node0 + (wave + u * (kBlock / 64)) * kSegNodes + lane;
+            const bool live = mh[u] != 0;
+            ml[u] = live ? g[kLineMLo * gp + idx] : 0.f;
+            pxh[u] = live ? g[kLinePxHi * gp + idx] : 0.f;
+            pxl[u] = live ? g[kLinePxLo * gp + idx] : 0.f;
+            pyh[u] = live ? g[kLinePyHi * gp + idx] : 0.f;
+            pyl[u] = live ? g[kLinePyLo * gp + idx] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int sl = wave + u * (kBlock / 64);
+            const int k = sl * kSegNodes + lane;
+            const long long idx = (long long)nc.node0 + k;
+            if (fo[u]) {
+                if (k < nc.count) {
 #pragma unroll
                     for (int l = 0; l < 6; l++) go[l * gp + idx] = 0;
                 }
-                if (mh[u] == 0) continue;
-                const double mass = (double)mh[u] + (double)ml[u];
-                const double px = (double)pxh[u] + (double)pxl[u];
-                const double py = (double)pyh[u] + (double)pyl[u];
-                const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
-                double vx, vy;
-                fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
-                g[kLinePxHi * gp + idx] = (float)vx;
-                g[kLinePyHi * gp + idx] = (float)vy;
+                if (lane == 0) flo[sl] = 0;
             }
+            if (mh[u] == 0) continue;
+            const double mass = (double)mh[u] + (double)ml[u];
+            const double px = (double)pxh[u] + (double)pxl[u];
+            const double py = (double)pyh[u] + (double)pyl[u];
+            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+            double vx, vy;
+            fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
+            g[kLinePxHi * gp + idx] = (float)vx;
+            g[kLinePyHi * gp + idx] = (float)vy;
         }
     }
 }

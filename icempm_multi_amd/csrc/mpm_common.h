@@ -8,6 +8,8 @@
 
 #include <stdint.h>
 
+#include <hip/hip_runtime.h>
+
 namespace icempm {
 
 constexpr int kNumPtsLines = 16;     // HostSideSOA lines (parameters_sim.h:27-35)
@@ -15,6 +17,11 @@ constexpr int kLineUtility = 0, kLineP = 1, kLineQ = 2, kLineJp = 3, kLinePosX =
               kLineFe00 = 8, kLineBp00 = 12;
 constexpr int64_t kCrushed = 0x10000, kDisabled = 0x20000, kLiquid = 0x40000;
 constexpr int kMaxIndenterSub = 1024;
+// Grid nodes are tracked in segments of 64 consecutive nodes (one wave): a
+// per-segment flag says that a buffer may hold mass there, so the grid update
+// reads and clears only flagged segments.
+constexpr int kSegShift = 6, kSegNodes = 1 << kSegShift;
+constexpr int kUpdateChunk = 1024;   // nodes per grid-update work item (16 segments)
 
 // Everything a kernel needs to know about one scene's physics and geometry.
 struct SceneConsts {
@@ -44,6 +51,7 @@ struct SceneDev {
     int n_tiles_x, n_tiles_y;
     int pad0;
     long long force_off;    // first of 2*n_ind_sub doubles in StepArgs::force_mem
+    long long seg_off;      // first segment flag of the scene in each half of StepArgs::dirty
 };
 
 // Device clock of a scene, advanced on the device once per substep so that a
@@ -91,6 +99,14 @@ struct StepArgs {
     int lane_spread;          // lane-to-point map of the tile kernels: 0 identity, 1 over 16 cells, 2 over 32 cells
     int dyn_tiles;            // 1: tiles pulled from per-XCD atomic queues, 0: static round robin
     int *tile_queue;          // 8 queue counters + 1 finished-block counter, zero between launches
+    int *dirty;               // segment flags: [buffer 0 of all scenes][buffer 1 of all scenes]
+    long long dirty_stride;   // segments per buffer (all scenes)
 };
+
+// Flags the segment of node idx of buffer buf of scene S as holding mass.
+__device__ __forceinline__ void mark_dirty(const StepArgs &a, const SceneDev &S, int buf, long long idx)
+{
+    a.dirty[buf * a.dirty_stride + S.seg_off + (idx >> kSegShift)] = 1;
+}
 
 }  // namespace icempm
