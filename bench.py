@@ -44,6 +44,25 @@ BYTES = {
 }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sheet4m_fused_tiles_hbm_traffic.md")
+
+
+def measured_traffic(precision_name, n_points):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated), or None
+    when the summary is absent or was taken on another point count."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    kv = dict(t.split("=") for t in line.split()[1:])
+                    if int(kv.get("points", -1)) == n_points and precision_name in kv:
+                        return int(float(kv[precision_name]))
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -273,7 +292,8 @@ def main():
             algo = n * B["fused_point"] + g_act * B["fused_node"]
             achieved = algo / t_launch / 1e9
             roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                        "frac": round(achieved * 1e9 / PEAK_HBM, 4), "traffic": None,
+                        "frac": round(achieved * 1e9 / PEAK_HBM, 4),
+                        "traffic": measured_traffic(precision_name, n),
                         "kernel": f"{precision_name}::k_tiles<3> (fused G2P+P2G)",
                         "avg_launch_us": round(t_launch * 1e6, 2), "algo_bytes_per_launch": algo,
                         "bytes_per_point": B["fused_point"], "bytes_per_node": B["fused_node"],
