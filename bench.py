@@ -258,7 +258,7 @@ def main():
             chosen = args.tile
         else:
             chosen, tune_ms = part.autotune(3)
-            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:25]]}")
+            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:15]]}")
         frame = P.UpdateEveryNthStep
         flags_timed = im.STEP_TIMINGS if not args.graph else im.STEP_GRAPH
 

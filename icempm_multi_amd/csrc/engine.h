@@ -94,7 +94,6 @@ private:
     int build_scene_dev(int i, SceneDev &d) const;
     int push_scenes();
     int push_clocks();
-    int pull_clocks();
     int build_tiles();
     int alloc_bins();
     StepArgs args() const;
@@ -124,6 +123,7 @@ private:
     unsigned *d_err_ = nullptr;
     int *d_tile_queue_ = nullptr;
     int *d_dirty_ = nullptr;   // 2 * n_segs_ segment flags
+    int clock_slot_ = 0;       // slot of d_clocks_ (2 x nscenes) holding the current clock
     long long n_segs_ = 0;
     NodeChunk *d_chunks_ = nullptr;
     int n_chunks_ = 0;

@@ -240,20 +240,9 @@ int Engine::push_clocks()
         ck[i].step = s.sim_step;
         ck[i].next_step = s.sim_step;
     }
-    HIPCHK(hipMemcpyAsync(d_clocks_, ck.data(), sizeof(SceneClock) * nscenes(), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(d_clocks_ + clock_slot_ * nscenes(), ck.data(), sizeof(SceneClock) * nscenes(),
+                          hipMemcpyHostToDevice, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    return ICEMPM_OK;
-}
-
-int Engine::pull_clocks()
-{
-    std::vector<SceneClock> ck(nscenes());
-    HIPCHK(hipMemcpyAsync(ck.data(), d_clocks_, sizeof(SceneClock) * nscenes(), hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
-    for (int i = 0; i < nscenes(); i++) {
-        scenes_[i].sim_time = ck[i].t;
-        scenes_[i].sim_step = ck[i].next_step;
-    }
     return ICEMPM_OK;
 }
 
@@ -317,11 +306,12 @@ int Engine::commit()
     const size_t buf_bytes = (size_t)gelems * eb;
     HIPCHK(hipMalloc(&grid_mem_, 2 * buf_bytes));
     HIPCHK(hipMemsetAsync(grid_mem_, 0, 2 * buf_bytes, stream_));
-    // segment flags (grid_elems is a multiple of the segment size)
+    // segment flags (grid_elems is a multiple of the segment size); each scene's
+    // range starts 16-byte aligned for the zero-fill kernel
     n_segs_ = 0;
     for (int i = 0; i < nscenes(); i++) {
         scenes_[i].seg_off = n_segs_;
-        n_segs_ += scenes_[i].grid_elems >> kSegShift;
+        n_segs_ += round_up(scenes_[i].grid_elems >> kSegShift, 4);
     }
     if (d_dirty_) { hipFree(d_dirty_); d_dirty_ = nullptr; }
     HIPCHK(hipMalloc(&d_dirty_, sizeof(int) * 2 * std::max<long long>(n_segs_, 1)));
@@ -342,7 +332,8 @@ int Engine::commit()
     if (d_clocks_) hipFree(d_clocks_);
     if (d_err_) hipFree(d_err_);
     HIPCHK(hipMalloc(&d_scenes_, sizeof(SceneDev) * nscenes()));
-    HIPCHK(hipMalloc(&d_clocks_, sizeof(SceneClock) * nscenes()));
+    HIPCHK(hipMalloc(&d_clocks_, sizeof(SceneClock) * 2 * nscenes()));   // two slots
+    clock_slot_ = 0;
     HIPCHK(hipMalloc(&d_err_, sizeof(unsigned) * nscenes()));
     HIPCHK(hipMemsetAsync(d_err_, 0, sizeof(unsigned) * nscenes(), stream_));
     if (!d_tile_queue_) {
@@ -418,7 +409,8 @@ StepArgs Engine::args() const
     a.scenes = d_scenes_;
     a.grid_mem = (char *)grid_mem_;
     a.force_mem = ind_force_mem_;
-    a.clocks = d_clocks_;
+    a.clocks = d_clocks_ + clock_slot_ * nscenes();
+    a.clocks_next = d_clocks_ + (clock_slot_ ^ 1) * nscenes();
     a.pts = pts_[0];
     a.pitch = pitch_;
     a.cell_start = bins_.cell_start;
@@ -679,7 +671,8 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
     ck.ind_y = ind_y;
     ck.step = scenes_[scene].sim_step;
     ck.next_step = scenes_[scene].sim_step;
-    HIPCHK(hipMemcpyAsync(d_clocks_ + scene, &ck, sizeof ck, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(d_clocks_ + clock_slot_ * nscenes() + scene, &ck, sizeof ck, hipMemcpyHostToDevice,
+                          stream_));
     StepArgs a = args();
     // only this scene's chunks: the chunk table is scene-major
     const NodeChunk *first = nullptr;
@@ -697,7 +690,7 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
     a.chunks = first;
     a.n_chunks = nchunks;
     timed_begin(true);
-    HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, std::min(nchunks, n_cus_ * 8), stream_));
+    HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, 0, std::min(nchunks, n_cus_ * 8), stream_));
     timed_end(true, kClsUpdate);
     HIPCHK(hipStreamSynchronize(stream_));   // ck lives on this stack frame
     return ICEMPM_OK;
@@ -854,14 +847,22 @@ int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *ce
 // ---------------------------------------------------------------------------
 int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed)
 {
-    const StepArgs a = args();
+    // clocks: the update of substep k reads slot cs and writes the advanced
+    // clock to slot cs^1, which the later launches of the substep read
+    StepArgs a = args();
+    SceneClock *slot[2] = {d_clocks_, d_clocks_ + nscenes()};
+    int cs = clock_slot_;
     const int nb_tiles = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
     const int nb_upd = std::min(n_chunks_, n_cus_ * 8);
     int b = start_buf;
     for (int k = 0; k < nsub; k++) {
-        timed_begin(timed);
-        HIPCHK(launch_clock(a, k == 0, stream_));
-        timed_end(timed, kClsClock);
+        StepArgs au = a;
+        au.clocks = slot[cs];
+        au.clocks_next = slot[cs ^ 1];
+        StepArgs an = a;               // after the update: the new clock
+        an.clocks = slot[cs ^ 1];
+        an.clocks_next = slot[cs];
+        cs ^= 1;
         if (direct) {
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++) {
@@ -874,36 +875,37 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
             timed_end(timed, kClsZero);
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++)
-                HIPCHK(LAUNCH_PREC(launch_p2g_direct, a, i, scenes_[i].n_pts, 0, stream_));
+                HIPCHK(LAUNCH_PREC(launch_p2g_direct, au, i, scenes_[i].n_pts, 0, stream_));
             timed_end(timed, kClsP2G);
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, nb_upd, stream_));
+            HIPCHK(LAUNCH_PREC(launch_update, au, 0, 0, 1, nb_upd, stream_));
             timed_end(timed, kClsUpdate);
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++)
-                HIPCHK(LAUNCH_PREC(launch_g2p_direct, a, i, scenes_[i].n_pts, 0, -1, stream_));
+                HIPCHK(LAUNCH_PREC(launch_g2p_direct, an, i, scenes_[i].n_pts, 0, -1, stream_));
             timed_end(timed, kClsG2P);
             continue;
         }
         if (k == 0 || !fused) {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, b, b, -1, nb_tiles, stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, nb_tiles, stream_));
             timed_end(timed, kClsP2G);
         }
         timed_begin(timed);
-        HIPCHK(LAUNCH_PREC(launch_update, a, b, 1, nb_upd, stream_));
+        HIPCHK(LAUNCH_PREC(launch_update, au, b, 1, 1, nb_upd, stream_));
         timed_end(timed, kClsUpdate);
         if (fused && k < nsub - 1) {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileFused, b, b ^ 1, -1, nb_tiles, stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileFused, b, b ^ 1, -1, nb_tiles, stream_));
             timed_end(timed, kClsFused);
         } else {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, b, b, -1, nb_tiles, stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, -1, nb_tiles, stream_));
             timed_end(timed, kClsG2P);
         }
         b ^= 1;
     }
+    clock_slot_ = cs;
     return ICEMPM_OK;
 }
 
@@ -929,8 +931,9 @@ int Engine::step(int nsub, int flags)
         }
     }
     HIPCHK(hipEventRecord(ev_a_, stream_));
+    const int slot0 = clock_slot_;
     if (use_graph) {
-        const auto key = std::make_tuple(nsub, start_buf, (direct ? 2 : 0) | (fused ? 1 : 0));
+        const auto key = std::make_tuple(nsub, start_buf, (direct ? 2 : 0) | (fused ? 1 : 0) | (slot0 << 2));
         auto it = graphs_.find(key);
         if (it == graphs_.end()) {
             HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -949,6 +952,7 @@ int Engine::step(int nsub, int flags)
     } else {
         RC(launch_substeps(nsub, start_buf, fused, direct, timed));
     }
+    clock_slot_ = slot0 ^ (nsub & 1);   // a replayed graph advances the slot like a direct launch
     HIPCHK(hipEventRecord(ev_b_, stream_));
     HIPCHK(hipEventSynchronize(ev_b_));
     HIPCHK(hipEventElapsedTime(&last_step_ms_, ev_a_, ev_b_));
@@ -977,12 +981,13 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
 {
     RC(ensure_committed());
     if (!bins_valid_) RC(sort());
-    // candidates: tile shape (tx, ty) x lane spread x tile distribution (static, dynamic)
+    // candidates: tile shape (tx, ty) x lane spread, static tile distribution.
+    // The dynamic queues (set_dyn_tiles) are not tried: on MI355X their
+    // device-scope atomics on one counter serialise and the launch takes
+    // about twice as long at every shape (measured on the 4M-point sheet).
     static const int cand[][4] = {{4, 32, 0, 0}, {4, 64, 0, 0}, {8, 16, 0, 0}, {8, 32, 0, 0}, {16, 16, 0, 0},
                                   {4, 32, 1, 0}, {4, 64, 1, 0}, {8, 16, 1, 0}, {8, 32, 1, 0}, {16, 16, 1, 0},
-                                  {4, 32, 2, 0}, {4, 64, 2, 0}, {8, 16, 2, 0}, {8, 32, 2, 0}, {16, 16, 2, 0},
-                                  {4, 32, 1, 1}, {4, 64, 1, 1}, {8, 16, 1, 1}, {8, 32, 1, 1}, {16, 16, 1, 1},
-                                  {4, 32, 2, 1}, {4, 64, 2, 1}, {8, 16, 2, 1}, {8, 32, 2, 1}, {16, 16, 2, 1}};
+                                  {4, 32, 2, 0}, {4, 64, 2, 0}, {8, 16, 2, 0}, {8, 32, 2, 0}, {16, 16, 2, 0}};
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
     // backup: points into the sort buffer, clocks and forces on the host

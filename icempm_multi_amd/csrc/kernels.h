@@ -16,7 +16,7 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int rec
                         hipStream_t s);
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
-hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int n_blocks, hipStream_t s);
+hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 }  // namespace fp64
 
@@ -26,13 +26,12 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int rec
                         hipStream_t s);
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
-hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int n_blocks, hipStream_t s);
+hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 }  // namespace mixed
 
 // kernels_common.hip
 // Clock of every scene: t += dt; ind_x = x0 + t*V; step counter (model.cpp:39-41).
-hipError_t launch_clock(const StepArgs &a, int first_substep_of_call, hipStream_t s);
 hipError_t launch_fill_zero(void *ptr, size_t bytes, hipStream_t s);
 
 // binning.hip -- device RemoveDisabledAndSort and tile tables.

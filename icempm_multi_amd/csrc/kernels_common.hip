@@ -1,6 +1,4 @@
 // kernels_common.hip -- small kernels shared by both precision forms.
-// Built with -ffp-contract=off: the clock must reproduce the host's
-// t += dt; indenter_x = x0 + t*V (model.cpp:39-41) bit for bit.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -8,20 +6,6 @@
 namespace icempm {
 
 namespace {
-
-__global__ void k_clock(StepArgs a)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.nscenes) return;
-    const SceneConsts &c = a.scenes[s].c;
-    SceneClock k = a.clocks[s];
-    k.t += c.dt;
-    k.ind_x = c.ind_x0 + k.t * c.ind_vel;
-    k.ind_y = c.ind_y;
-    k.step = k.next_step;
-    k.next_step = k.step + 1;
-    a.clocks[s] = k;
-}
 
 __global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, size_t ntail)
 {
@@ -32,13 +16,6 @@ __global__ void k_fill_zero(uint4 *p, size_t n16, unsigned char *tail, size_t nt
 }
 
 }  // namespace
-
-hipError_t launch_clock(const StepArgs &a, int first_substep_of_call, hipStream_t s)
-{
-    (void)first_substep_of_call;
-    hipLaunchKernelGGL(k_clock, dim3((a.nscenes + 63) / 64), dim3(64), 0, s, a);
-    return hipGetLastError();
-}
 
 // Zero-fills `bytes` bytes with a kernel (graph-safe, unlike a memset node).
 // ptr must be 16-byte aligned.

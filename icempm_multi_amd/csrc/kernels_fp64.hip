@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 // Grid update (kernels.cuh:164-301, live lines), one thread per node.  With
 // zero_other set it also clears the other accumulation grid for the next P2G.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
+__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other, int advance_clock)
 {
     // Each wave takes 4 of the chunk's 16 segments of 64 nodes, one node per
     // lane.  A segment of this buffer is read only if flagged; a flagged
@@ -251,7 +251,8 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         const NodeChunk nc = a.chunks[ch];
         const SceneDev &S = a.scenes[nc.scene];
         const SceneConsts &c = S.c;
-        const SceneClock clk = a.clocks[nc.scene];
+        SceneClock clk = a.clocks[nc.scene];
+        if (advance_clock) clk = next_clock(c, clk);
         double *g = (double *)(a.grid_mem + S.grid_off[buf]);
         double *go = (double *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
@@ -300,6 +301,10 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
             g[2 * gp + idx] = vy;
         }
     }
+    // the advanced clocks go to the other slot, which no block reads here
+    if (advance_clock && blockIdx.x == 0)
+        for (int sc = threadIdx.x; sc < a.nscenes; sc += kBlock)
+            a.clocks_next[sc] = next_clock(a.scenes[sc].c, a.clocks[sc]);
 }
 
 // ---------------------------------------------------------------------------
@@ -333,10 +338,10 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int rec
     return hipGetLastError();
 }
 
-hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int n_blocks, hipStream_t s)
+hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s)
 {
-    if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_update, dim3(n_blocks), dim3(kBlock), 0, s, a, buf, zero_other);
+    if (n_blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_update, dim3(n_blocks), dim3(kBlock), 0, s, a, buf, zero_other, advance_clock);
     return hipGetLastError();
 }
 

@@ -222,7 +222,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 }
 
 // Grid update (kernels.cuh:164-301) in FP64 on hi + lo; velocity stored in FP32.
-__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other)
+__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other, int advance_clock)
 {
     // segments of 64 nodes as in the FP64 form; 6 FP32 lines per buffer
     constexpr int kU = kUpdateChunk / kSegNodes / (kBlock / 64);   // 4
@@ -231,7 +231,8 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
         const NodeChunk nc = a.chunks[ch];
         const SceneDev &S = a.scenes[nc.scene];
         const SceneConsts &c = S.c;
-        const SceneClock clk = a.clocks[nc.scene];
+        SceneClock clk = a.clocks[nc.scene];
+        if (advance_clock) clk = next_clock(c, clk);
         glb_f *g = (glb_f *)(a.grid_mem + S.grid_off[buf]);
         glb_f *go = (glb_f *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
@@ -287,6 +288,10 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
             g[kLinePyHi * gp + idx] = (float)vy;
         }
     }
+    // the advanced clocks go to the other slot, which no block reads here
+    if (advance_clock && blockIdx.x == 0)
+        for (int sc = threadIdx.x; sc < a.nscenes; sc += kBlock)
+            a.clocks_next[sc] = next_clock(a.scenes[sc].c, a.clocks[sc]);
 }
 
 size_t tile_lds_bytes(const StepArgs &a, int mode)
@@ -319,10 +324,10 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int rec
     return hipGetLastError();
 }
 
-hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int n_blocks, hipStream_t s)
+hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s)
 {
-    if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_update, dim3(n_blocks), dim3(kBlock), 0, s, a, buf, zero_other);
+    if (n_blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_update, dim3(n_blocks), dim3(kBlock), 0, s, a, buf, zero_other, advance_clock);
     return hipGetLastError();
 }
 

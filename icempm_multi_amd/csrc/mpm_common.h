@@ -84,7 +84,8 @@ struct StepArgs {
     const SceneDev *scenes;
     char *grid_mem;           // base of every scene's grids (a kernel argument keeps it in the global space)
     double *force_mem;        // base of every scene's indenter force bins
-    SceneClock *clocks;
+    SceneClock *clocks;       // clock of the current substep, per scene
+    SceneClock *clocks_next;  // the other clock slot: the grid update writes the next substep's clock here
     void *pts;                // SoA base: kNumPtsLines lines of `pitch` elements
     long long pitch;
     const int *cell_start;    // per scene ncells+1 global point indices (from the last sort)
@@ -102,6 +103,19 @@ struct StepArgs {
     int *dirty;               // segment flags: [buffer 0 of all scenes][buffer 1 of all scenes]
     long long dirty_stride;   // segments per buffer (all scenes)
 };
+
+// The clock of the next substep (model.cpp:39-41): t += dt by repeated
+// addition, indenter_x = x0 + t*V, each operation rounded on its own so that
+// the device clock equals the host's bit for bit.
+__device__ __forceinline__ SceneClock next_clock(const SceneConsts &c, SceneClock k)
+{
+    k.t = __dadd_rn(k.t, c.dt);
+    k.ind_x = __dadd_rn(c.ind_x0, __dmul_rn(k.t, c.ind_vel));
+    k.ind_y = c.ind_y;
+    k.step = k.next_step;
+    k.next_step = k.step + 1;
+    return k;
+}
 
 // Flags the segment of node idx of buffer buf of scene S as holding mass.
 __device__ __forceinline__ void mark_dirty(const StepArgs &a, const SceneDev &S, int buf, long long idx)
