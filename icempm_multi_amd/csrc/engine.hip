@@ -344,9 +344,9 @@ int Engine::commit()
 
     std::vector<NodeChunk> chunks;
     for (int i = 0; i < nscenes(); i++) {
-        const icempm_simparams &P = scenes_[i].prms;
-        const int nodes = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY;
-        for (int n0 = 0; n0 < nodes; n0 += kUpdateChunk) chunks.push_back({i, n0, std::min(kUpdateChunk, nodes - n0), 0});
+        const int nseg = (int)(scenes_[i].grid_elems >> kSegShift);
+        const int len = update_run_length(scenes_[i].grid_elems);
+        for (int s0 = 0; s0 < nseg; s0 += len) chunks.push_back({i, s0, std::min(len, nseg - s0), 0});
     }
     if (d_chunks_) hipFree(d_chunks_);
     n_chunks_ = (int)chunks.size();
@@ -680,9 +680,9 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
     {
         long long k = 0;
         for (int i = 0; i < nscenes(); i++) {
-            const icempm_simparams &P = scenes_[i].prms;
-            const long long nodes = (long long)(P.GridXTotal + 2 * P.GridHaloSize) * P.GridY;
-            const int nc = (int)((nodes + kUpdateChunk - 1) / kUpdateChunk);
+            const long long nseg = scenes_[i].grid_elems >> kSegShift;
+            const int len = update_run_length(scenes_[i].grid_elems);
+            const int nc = (int)((nseg + len - 1) / len);
             if (i == scene) { first = d_chunks_ + k; nchunks = nc; }
             k += nc;
         }
@@ -690,7 +690,7 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
     a.chunks = first;
     a.n_chunks = nchunks;
     timed_begin(true);
-    HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, 0, std::min(nchunks, n_cus_ * 8), stream_));
+    HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, 0, update_blocks(nchunks), stream_));
     timed_end(true, kClsUpdate);
     HIPCHK(hipStreamSynchronize(stream_));   // ck lives on this stack frame
     return ICEMPM_OK;
@@ -853,7 +853,7 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
     SceneClock *slot[2] = {d_clocks_, d_clocks_ + nscenes()};
     int cs = clock_slot_;
     const int nb_tiles = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
-    const int nb_upd = std::min(n_chunks_, n_cus_ * 8);
+    const int nb_upd = update_blocks(n_chunks_);
     int b = start_buf;
     for (int k = 0; k < nsub; k++) {
         StepArgs au = a;

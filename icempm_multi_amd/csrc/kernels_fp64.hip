@@ -236,69 +236,74 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 }
 
 // ---------------------------------------------------------------------------
-// Grid update (kernels.cuh:164-301, live lines), one thread per node.  With
-// zero_other set it also clears the other accumulation grid for the next P2G.
+// Grid update (kernels.cuh:164-301, live lines), one lane per node of the
+// flagged segments.  With zero_other set it also clears the other
+// accumulation grid for the next P2G; with advance_clock set it computes the
+// substep's clock from the current slot and block 0 writes it to the next.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other, int advance_clock)
 {
-    // Each wave takes 4 of the chunk's 16 segments of 64 nodes, one node per
-    // lane.  A segment of this buffer is read only if flagged; a flagged
-    // segment of the other buffer is cleared (all 64 nodes) for the next P2G
-    // and its flag reset.  The loads of the 4 segments are in flight together.
-    constexpr int kU = kUpdateChunk / kSegNodes / (kBlock / 64);   // 4
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
-        const NodeChunk nc = a.chunks[ch];
-        const SceneDev &S = a.scenes[nc.scene];
+    // A wave takes a run of up to 64 node segments: lane l loads the flags of
+    // segment l of the run for both buffers, a ballot gives the flagged ones.
+    // Flagged segments of the other buffer are cleared (all 64 nodes, one per
+    // lane) and their flags reset; flagged segments of this buffer are
+    // updated, 8 at a time with their loads in flight together.
+    constexpr int kU = 8;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (kBlock / 64);
+    for (int r = wave; r < a.n_chunks; r += n_waves) {
+        const NodeChunk run = a.chunks[r];
+        const SceneDev &S = a.scenes[run.scene];
         const SceneConsts &c = S.c;
-        SceneClock clk = a.clocks[nc.scene];
+        SceneClock clk = a.clocks[run.scene];
         if (advance_clock) clk = next_clock(c, clk);
         double *g = (double *)(a.grid_mem + S.grid_off[buf]);
         double *go = (double *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
-        int *fl = a.dirty + buf * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
-        int *flo = a.dirty + (buf ^ 1) * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
-        int f[kU], fo[kU];
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int sl = wave + u * (kBlock / 64);
-            const bool in = sl * kSegNodes < nc.count;
-            f[u] = in ? fl[sl] : 0;
-            fo[u] = (in && zero_other) ? flo[sl] : 0;
+        int *fl = a.dirty + buf * a.dirty_stride + S.seg_off + run.seg0;
+        int *flo = a.dirty + (buf ^ 1) * a.dirty_stride + S.seg_off + run.seg0;
+        const bool in = lane < run.nseg;
+        const int f = in ? fl[lane] : 0;
+        const int fo = (in && zero_other) ? flo[lane] : 0;
+        unsigned long long mb = __ballot(f != 0), mo = __ballot(fo != 0);
+        const long long node0 = (long long)run.seg0 << kSegShift;
+        while (mo) {
+            const int sg = __builtin_ctzll(mo);
+            mo &= mo - 1;
+            const long long idx = node0 + ((long long)sg << kSegShift) + lane;
+            go[idx] = 0;
+            go[gp + idx] = 0;
+            go[2 * gp + idx] = 0;
         }
-        double m[kU], px[kU], py[kU];
+        if (fo) flo[lane] = 0;
+        while (mb) {
+            int sg[kU];
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int k = (wave + u * (kBlock / 64)) * kSegNodes + lane;
-            const long long idx = (long long)nc.node0 + k;
-            m[u] = (f[u] && k < nc.count) ? g[idx] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const long long idx = (long long)nc.node0 + (wave + u * (kBlock / 64)) * kSegNodes + lane;
-            px[u] = m[u] != 0 ? g[gp + idx] : 0.0;
-            py[u] = m[u] != 0 ? g[2 * gp + idx] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int sl = wave + u * (kBlock / 64);
-            const int k = sl * kSegNodes + lane;
-            const long long idx = (long long)nc.node0 + k;
-            if (fo[u]) {
-                if (k < nc.count) {
-                    go[idx] = 0;
-                    go[gp + idx] = 0;
-                    go[2 * gp + idx] = 0;
-                }
-                if (lane == 0) flo[sl] = 0;
+            for (int u = 0; u < kU; u++) {
+                sg[u] = mb ? __builtin_ctzll(mb) : -1;
+                mb &= mb - 1;
             }
-            if (m[u] == 0) continue;
-            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
-            double vx, vy;
-            node_velocity(c, clk, force, gix, giy, m[u], px[u], py[u], vx, vy);
-            g[gp + idx] = vx;
-            g[2 * gp + idx] = vy;
+            double m[kU], px[kU], py[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                m[u] = sg[u] >= 0 ? g[node0 + ((long long)sg[u] << kSegShift) + lane] : 0.0;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const long long idx = node0 + ((long long)sg[u] << kSegShift) + lane;
+                px[u] = m[u] != 0 ? g[gp + idx] : 0.0;
+                py[u] = m[u] != 0 ? g[2 * gp + idx] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                if (m[u] == 0) continue;
+                const long long idx = node0 + ((long long)sg[u] << kSegShift) + lane;
+                const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+                double vx, vy;
+                node_velocity(c, clk, force, gix, giy, m[u], px[u], py[u], vx, vy);
+                g[gp + idx] = vx;
+                g[2 * gp + idx] = vy;
+            }
         }
     }
     // the advanced clocks go to the other slot, which no block reads here

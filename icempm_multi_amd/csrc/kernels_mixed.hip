@@ -224,68 +224,70 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 // Grid update (kernels.cuh:164-301) in FP64 on hi + lo; velocity stored in FP32.
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other, int advance_clock)
 {
-    // segments of 64 nodes as in the FP64 form; 6 FP32 lines per buffer
-    constexpr int kU = kUpdateChunk / kSegNodes / (kBlock / 64);   // 4
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int ch = blockIdx.x; ch < a.n_chunks; ch += gridDim.x) {
-        const NodeChunk nc = a.chunks[ch];
-        const SceneDev &S = a.scenes[nc.scene];
+    // runs of 64 node segments, one per wave, as in the FP64 form; 6 FP32 lines per buffer
+    constexpr int kU = 8;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (kBlock / 64);
+    for (int r = wave; r < a.n_chunks; r += n_waves) {
+        const NodeChunk run = a.chunks[r];
+        const SceneDev &S = a.scenes[run.scene];
         const SceneConsts &c = S.c;
-        SceneClock clk = a.clocks[nc.scene];
+        SceneClock clk = a.clocks[run.scene];
         if (advance_clock) clk = next_clock(c, clk);
         glb_f *g = (glb_f *)(a.grid_mem + S.grid_off[buf]);
         glb_f *go = (glb_f *)(a.grid_mem + S.grid_off[buf ^ 1]);
         double *force = a.force_mem + S.force_off;
         const long long gp = S.gpitch;
-        int *fl = a.dirty + buf * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
-        int *flo = a.dirty + (buf ^ 1) * a.dirty_stride + S.seg_off + (nc.node0 >> kSegShift);
-        int f[kU], fo[kU];
+        int *fl = a.dirty + buf * a.dirty_stride + S.seg_off + run.seg0;
+        int *flo = a.dirty + (buf ^ 1) * a.dirty_stride + S.seg_off + run.seg0;
+        const bool in = lane < run.nseg;
+        const int f = in ? fl[lane] : 0;
+        const int fo = (in && zero_other) ? flo[lane] : 0;
+        unsigned long long mb = __ballot(f != 0), mo = __ballot(fo != 0);
+        const long long node0 = (long long)run.seg0 << kSegShift;
+        while (mo) {
+            const int sg = __builtin_ctzll(mo);
+            mo &= mo - 1;
+            const long long idx = node0 + ((long long)sg << kSegShift) + lane;
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int sl = wave + u * (kBlock / 64);
-            const bool in = sl * kSegNodes < nc.count;
-            f[u] = in ? fl[sl] : 0;
-            fo[u] = (in && zero_other) ? flo[sl] : 0;
+            for (int l = 0; l < 6; l++) go[l * gp + idx] = 0;
         }
-        float mh[kU];
+        if (fo) flo[lane] = 0;
+        while (mb) {
+            int sg[kU];
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int k = (wave + u * (kBlock / 64)) * kSegNodes + lane;
-            const long long idx = (long long)nc.node0 + k;
-            mh[u] = (f[u] && k < nc.count) ? g[idx] : 0.f;
-        }
-        float ml[kU], pxh[kU], pxl[kU], pyh[kU], pyl[kU];
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const long long idx = (long long)nc.node0 + (wave + u * (kBlock / 64)) * kSegNodes + lane;
-            const bool live = mh[u] != 0;
-            ml[u] = live ? g[kLineMLo * gp + idx] : 0.f;
-            pxh[u] = live ? g[kLinePxHi * gp + idx] : 0.f;
-            pxl[u] = live ? g[kLinePxLo * gp + idx] : 0.f;
-            pyh[u] = live ? g[kLinePyHi * gp + idx] : 0.f;
-            pyl[u] = live ? g[kLinePyLo * gp + idx] : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int sl = wave + u * (kBlock / 64);
-            const int k = sl * kSegNodes + lane;
-            const long long idx = (long long)nc.node0 + k;
-            if (fo[u]) {
-                if (k < nc.count) {
-#pragma unroll
-                    for (int l = 0; l < 6; l++) go[l * gp + idx] = 0;
-                }
-                if (lane == 0) flo[sl] = 0;
+            for (int u = 0; u < kU; u++) {
+                sg[u] = mb ? __builtin_ctzll(mb) : -1;
+                mb &= mb - 1;
             }
-            if (mh[u] == 0) continue;
-            const double mass = (double)mh[u] + (double)ml[u];
-            const double px = (double)pxh[u] + (double)pxl[u];
-            const double py = (double)pyh[u] + (double)pyl[u];
-            const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
-            double vx, vy;
-            fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
-            g[kLinePxHi * gp + idx] = (float)vx;
-            g[kLinePyHi * gp + idx] = (float)vy;
+            float mh[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                mh[u] = sg[u] >= 0 ? g[node0 + ((long long)sg[u] << kSegShift) + lane] : 0.f;
+            float ml[kU], pxh[kU], pxl[kU], pyh[kU], pyl[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const long long idx = node0 + ((long long)sg[u] << kSegShift) + lane;
+                const bool live = mh[u] != 0;
+                ml[u] = live ? g[kLineMLo * gp + idx] : 0.f;
+                pxh[u] = live ? g[kLinePxHi * gp + idx] : 0.f;
+                pxl[u] = live ? g[kLinePxLo * gp + idx] : 0.f;
+                pyh[u] = live ? g[kLinePyHi * gp + idx] : 0.f;
+                pyl[u] = live ? g[kLinePyLo * gp + idx] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                if (mh[u] == 0) continue;
+                const long long idx = node0 + ((long long)sg[u] << kSegShift) + lane;
+                const double mass = (double)mh[u] + (double)ml[u];
+                const double px = (double)pxh[u] + (double)pxl[u];
+                const double py = (double)pyh[u] + (double)pyl[u];
+                const int gix = (int)(idx / c.gridY) - c.halo + c.xoff, giy = (int)(idx % c.gridY);
+                double vx, vy;
+                fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
+                g[kLinePxHi * gp + idx] = (float)vx;
+                g[kLinePyHi * gp + idx] = (float)vy;
+            }
         }
     }
     // the advanced clocks go to the other slot, which no block reads here

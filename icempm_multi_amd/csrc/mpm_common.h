@@ -21,7 +21,10 @@ constexpr int kMaxIndenterSub = 1024;
 // per-segment flag says that a buffer may hold mass there, so the grid update
 // reads and clears only flagged segments.
 constexpr int kSegShift = 6, kSegNodes = 1 << kSegShift;
-constexpr int kUpdateChunk = 1024;   // nodes per grid-update work item (16 segments)
+// Segments per grid-update work item (one wave; one flag per lane): short runs
+// spread dense grids over many waves, long runs let a wave skip a large empty
+// grid with one flag load per 64 segments.
+inline int update_run_length(long long grid_nodes) { return grid_nodes > (8LL << 20) ? 64 : 16; }
 
 // Everything a kernel needs to know about one scene's physics and geometry.
 struct SceneConsts {
@@ -71,11 +74,12 @@ struct TileItem {
     int npts;
 };
 
-// A contiguous run of grid nodes of one scene: the unit of the grid update.
+// A run of up to 64 node segments of one scene (up to 4096 nodes): the unit
+// of work of one wave in the grid update.
 struct NodeChunk {
     int scene;
-    int node0;
-    int count;
+    int seg0;     // first segment (node seg0*64)
+    int nseg;     // segments in the run, 1..64
     int pad;
 };
 
