@@ -182,6 +182,12 @@ def batch_workload(args, im, scenes):
             bt.add_scene(P, h, n)
         bt.commit()
         chosen, ms = bt.autotune(2)
+        log(f"[{args.workload} {name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in ms[:15]]}")
+        bt.reset_kernel_stats()
+        bt.step(20, im.STEP_TIMINGS)
+        st = bt.kernel_stats()
+        log(f"[{args.workload} {name}] kernel ms per launch: "
+            f"{ {k: round(v[0] / v[1], 4) for k, v in st.items() if v[1]} }")
         bt.step(args.warmup, im.STEP_GRAPH)
         bt.step(K, im.STEP_GRAPH)          # captures the K-substep graph
         t0 = time.perf_counter()

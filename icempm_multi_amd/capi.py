@@ -79,6 +79,8 @@ _SIGS = {
     "icempm_batch_get_error_code": (C.c_int, [_VP, C.POINTER(C.c_uint8)]),
     "icempm_batch_get_scene_count": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "icempm_batch_last_step_ms": (C.c_int, [_VP, C.POINTER(C.c_float)]),
+    "icempm_batch_get_kernel_stats": (C.c_int, [_VP, _D, C.POINTER(C.c_longlong), C.c_int]),
+    "icempm_batch_reset_kernel_stats": (C.c_int, [_VP]),
     "icempm_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
 }
 
@@ -357,6 +359,15 @@ class Batch:
         chosen = C.c_int(-1)
         _check(lib().icempm_batch_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
         return chosen.value, list(out)
+
+    def kernel_stats(self):
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = (C.c_longlong * len(KERNEL_CLASSES))()
+        _check(lib().icempm_batch_get_kernel_stats(self._h, _dptr(ms), cnt, len(KERNEL_CLASSES)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def reset_kernel_stats(self):
+        _check(lib().icempm_batch_reset_kernel_stats(self._h))
 
     def last_step_ms(self):
         v = C.c_float()

@@ -489,4 +489,16 @@ int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep)
     return ICEMPM_OK;
 }
 
+int icempm_batch_get_kernel_stats(icempm_batch *b, double *ms, long long *count, int n)
+{
+    if (!b) return bad("null batch");
+    return guarded([&] { return b->eng->get_kernel_stats(ms, count, n); });
+}
+
+int icempm_batch_reset_kernel_stats(icempm_batch *b)
+{
+    if (!b) return bad("null batch");
+    return guarded([&] { return b->eng->reset_kernel_stats(); });
+}
+
 }  // extern "C"

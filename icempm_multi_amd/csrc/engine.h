@@ -74,6 +74,7 @@ public:
     int get_error_code(uint8_t *code);
     int get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts, int n_cells);
     int get_kernel_stats(double *ms, long long *count, int n);
+    int reset_kernel_stats();
     int get_grid(int scene, int buf, double *out, long long n_nodes);
     int get_position_map(int scene, int *fb, int *ox, int *oy);
     const SceneHost &scene(int i) const { return scenes_[i]; }

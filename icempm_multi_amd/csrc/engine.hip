@@ -718,6 +718,11 @@ int Engine::reset_indenter_force()
     long long forces = 0;
     for (auto &s : scenes_) forces += 2LL * s.prms.n_indenter_subdivisions;
     HIPCHK(launch_fill_zero(ind_force_mem_, sizeof(double) * forces, stream_));
+    return reset_kernel_stats();
+}
+
+int Engine::reset_kernel_stats()
+{
     RC(collect_timings());
     for (int k = 0; k < kNumCls; k++) { cls_ms_[k] = 0; cls_cnt_[k] = 0; }
     return ICEMPM_OK;
@@ -985,9 +990,11 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     // The dynamic queues (set_dyn_tiles) are not tried: on MI355X their
     // device-scope atomics on one counter serialise and the launch takes
     // about twice as long at every shape (measured on the 4M-point sheet).
-    static const int cand[][4] = {{4, 32, 0, 0}, {4, 64, 0, 0}, {8, 16, 0, 0}, {8, 32, 0, 0}, {16, 16, 0, 0},
-                                  {4, 32, 1, 0}, {4, 64, 1, 0}, {8, 16, 1, 0}, {8, 32, 1, 0}, {16, 16, 1, 0},
-                                  {4, 32, 2, 0}, {4, 64, 2, 0}, {8, 16, 2, 0}, {8, 32, 2, 0}, {16, 16, 2, 0}};
+    // Lane spread 1 won at every shape on the 4M-point sheet, so the other maps
+    // are tried at two shapes only; the small tiles are for sparse scenes.
+    static const int cand[][4] = {{4, 16, 1, 0}, {8, 8, 1, 0}, {4, 32, 1, 0}, {4, 64, 1, 0}, {8, 16, 1, 0},
+                                  {8, 32, 1, 0}, {16, 16, 1, 0}, {4, 32, 0, 0}, {8, 32, 0, 0}, {4, 32, 2, 0},
+                                  {8, 32, 2, 0}};
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
     // backup: points into the sort buffer, clocks and forces on the host

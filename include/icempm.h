@@ -280,6 +280,10 @@ int icempm_batch_get_scene_count(icempm_batch *b, int *n_scenes, long long *n_po
 int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_out, int *chosen);
 /* Average device time per substep (ms) of the last icempm_batch_step call. */
 int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep);
+/* Device time per kernel class of the batch (classes as for partitions),
+ * recorded when ICEMPM_STEP_TIMINGS is set; reset by icempm_batch_reset_kernel_stats. */
+int icempm_batch_get_kernel_stats(icempm_batch *b, double *ms, long long *count, int n);
+int icempm_batch_reset_kernel_stats(icempm_batch *b);
 
 #ifdef __cplusplus
 }
