@@ -220,7 +220,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the sheet length (tests only)")
-    ap.add_argument("--cpu-substeps", type=int, default=30)
+    ap.add_argument("--cpu-substeps", type=int, default=300, help="substeps of the CPU baseline sample (~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay frames as hipGraphs (no per-kernel events)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
