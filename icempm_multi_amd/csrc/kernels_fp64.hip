@@ -82,12 +82,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         }
         const TileItem ti = a.tiles[w];
         const SceneDev &S = a.scenes[ti.scene];
-        const SceneConsts &c = S.c;
+        const SceneConsts c = uniform_copy(S.c);
         const int gridY = c.gridY;
         const int wc0 = ti.col0 - 1 - a.margin, wr0 = ti.row0 - 1 - a.margin;
         const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rbuf]);
         glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
+        int *const dflag = a.dirty + wbuf * a.dirty_stride + uniform_copy(S.seg_off);   // segment flags of wbuf
         bool record = false;
         if (DO_G2P)
             record = record_mode >= 0 ? record_mode != 0 : (a.clocks[ti.scene].step % c.update_every) == 0;
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                             glb_add(gw + gi, dm);
                             glb_add(gw + gp + gi, dpx);
                             glb_add(gw + 2 * gp + gi, dpy);
-                            mark_dirty(a, S, wbuf, gi);
+                            dflag[gi >> kSegShift] = 1;
                         }
                     });
                 err = max(err, e);
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                 glb_add(gw + gi, m);
                 glb_add(gw + gp + gi, apx[i]);
                 glb_add(gw + 2 * gp + gi, apy[i]);
-                mark_dirty(a, S, wbuf, gi);
+                dflag[gi >> kSegShift] = 1;
             }
         }
         __syncthreads();

@@ -70,13 +70,14 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         }
         const TileItem ti = a.tiles[w];
         const SceneDev &S = a.scenes[ti.scene];
-        const SceneConsts &c = S.c;
+        const SceneConsts c = uniform_copy(S.c);
         const ConstsF cf = to_float(c);
         const int gridY = c.gridY;
         const int wc0 = ti.col0 - 1 - a.margin, wr0 = ti.row0 - 1 - a.margin;
         const glb_f *gr = (const glb_f *)(a.grid_mem + S.grid_off[rbuf]);
         glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
+        int *const dflag = a.dirty + wbuf * a.dirty_stride + uniform_copy(S.seg_off);   // segment flags of wbuf
         bool record = false;
         const int step = a.clocks[ti.scene].step;
         if (DO_G2P) record = record_mode >= 0 ? record_mode != 0 : (step % c.update_every) == 0;
@@ -192,7 +193,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                             twosum_add(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm, 0.f);
                             twosum_add(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx, 0.f);
                             twosum_add(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy, 0.f);
-                            mark_dirty(a, S, wbuf, gi);
+                            dflag[gi >> kSegShift] = 1;
                         }
                     });
                 err = max(err, e);
@@ -210,7 +211,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, m);
                 twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, apx[i]);
                 twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, apy[i]);
-                mark_dirty(a, S, wbuf, gi);
+                dflag[gi >> kSegShift] = 1;
             }
         }
         __syncthreads();

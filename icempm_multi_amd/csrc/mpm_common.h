@@ -121,6 +121,22 @@ __device__ __forceinline__ SceneClock next_clock(const SceneConsts &c, SceneCloc
     return k;
 }
 
+// A copy of workgroup-uniform data held in scalar registers: every dword goes
+// through readfirstlane.  Read through a pointer instead, the constants would
+// be reloaded from memory after every store or atomic the kernel issues (the
+// compiler cannot rule out aliasing) and kept in vector registers.
+template <class T>
+__device__ __forceinline__ T uniform_copy(const T &src)
+{
+    static_assert(sizeof(T) % 4 == 0, "uniform_copy works on whole dwords");
+    T out;
+    const unsigned *s = reinterpret_cast<const unsigned *>(&src);
+    unsigned *d = reinterpret_cast<unsigned *>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+    return out;
+}
+
 // Flags the segment of node idx of buffer buf of scene S as holding mass.
 __device__ __forceinline__ void mark_dirty(const StepArgs &a, const SceneDev &S, int buf, long long idx)
 {
