@@ -242,7 +242,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 // accumulation grid for the next P2G; with advance_clock set it computes the
 // substep's clock from the current slot and block 0 writes it to the next.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other, int advance_clock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_update(StepArgs a, int buf, int zero_other,
+                                                                                            int advance_clock)
 {
     // A wave takes a run of up to 64 node segments: lane l loads the flags of
     // segment l of the run for both buffers, a ballot gives the flagged ones.
@@ -253,10 +254,10 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (kBlock / 64);
     for (int r = wave; r < a.n_chunks; r += n_waves) {
-        const NodeChunk run = a.chunks[r];
+        const NodeChunk run = uniform_copy(a.chunks[r]);
         const SceneDev &S = a.scenes[run.scene];
-        const SceneConsts &c = S.c;
-        SceneClock clk = a.clocks[run.scene];
+        const SceneConsts c = uniform_copy(S.c);
+        SceneClock clk = uniform_copy(a.clocks[run.scene]);
         if (advance_clock) clk = next_clock(c, clk);
         double *g = (double *)(a.grid_mem + S.grid_off[buf]);
         double *go = (double *)(a.grid_mem + S.grid_off[buf ^ 1]);

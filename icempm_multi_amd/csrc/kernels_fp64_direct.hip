@@ -16,7 +16,7 @@ namespace fp64 {
 __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, int wbuf)
 {
     const SceneDev &S = a.scenes[scene];
-    const SceneConsts &c = S.c;
+    const SceneConsts c = uniform_copy(S.c);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= S.n_pts) return;
     const long long pt = S.pt_off + i;
@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
 __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, int rbuf, int record)
 {
     const SceneDev &S = a.scenes[scene];
-    const SceneConsts &c = S.c;
+    const SceneConsts c = uniform_copy(S.c);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= S.n_pts) return;
     const long long pt = S.pt_off + i;

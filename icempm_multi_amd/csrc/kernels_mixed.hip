@@ -223,17 +223,18 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 }
 
 // Grid update (kernels.cuh:164-301) in FP64 on hi + lo; velocity stored in FP32.
-__global__ void __launch_bounds__(kBlock) k_update(StepArgs a, int buf, int zero_other, int advance_clock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_update(StepArgs a, int buf, int zero_other,
+                                                                                            int advance_clock)
 {
     // runs of 64 node segments, one per wave, as in the FP64 form; 6 FP32 lines per buffer
-    constexpr int kU = 8;
+    constexpr int kU = 4;   // 6 FP32 lines per node: 4 segments in flight fit 4 waves per SIMD
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (kBlock / 64);
     for (int r = wave; r < a.n_chunks; r += n_waves) {
-        const NodeChunk run = a.chunks[r];
+        const NodeChunk run = uniform_copy(a.chunks[r]);
         const SceneDev &S = a.scenes[run.scene];
-        const SceneConsts &c = S.c;
-        SceneClock clk = a.clocks[run.scene];
+        const SceneConsts c = uniform_copy(S.c);
+        SceneClock clk = uniform_copy(a.clocks[run.scene]);
         if (advance_clock) clk = next_clock(c, clk);
         glb_f *g = (glb_f *)(a.grid_mem + S.grid_off[buf]);
         glb_f *go = (glb_f *)(a.grid_mem + S.grid_off[buf ^ 1]);
