@@ -76,32 +76,38 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     v[0] = v[1] = 0;
     B[0] = B[1] = B[2] = B[3] = 0;
     const double s4 = 4. * c.hinv;
-    // the gather is left rolled in i: fully unrolled it keeps 18 node values
-    // live and the fused kernel drops to 2 waves per SIMD
+    auto gather = [&](int i, int j, double nvx, double nvy) {
+        const double d0 = i - fx, d1 = j - fy;
+        const double w = wx[i] * wy[j];
+        v[0] += w * nvx;
+        v[1] += w * nvy;
+        const double s = s4 * w;
+        B[0] += s * (nvx * d0);
+        B[1] += s * (nvx * d1);
+        B[2] += s * (nvy * d0);
+        B[3] += s * (nvy * d1);
+    };
+    // Two separate loops: if the LDS and the global reads fed one loop, the
+    // merged value would make every gather wait for all outstanding global
+    // loads, including the prefetch of the next point.  The gather is left
+    // rolled in i: fully unrolled it keeps 18 node values live.
+    if (inwin) {
 #pragma unroll 1
-    for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            double nvx, nvy;
-            if (inwin) {
+            for (int j = 0; j < 3; j++) {
                 const int li = (lx + i) * win_h + ly + j;
-                nvx = vw[2 * li];           // (vx, vy) interleaved: one 128-bit LDS read
-                nvy = vw[2 * li + 1];
-            } else {
-                const long long gi = (long long)(bxc + i) * c.gridY + by + j;
-                nvx = g[gpitch + gi];
-                nvy = g[2 * gpitch + gi];
+                gather(i, j, vw[2 * li], vw[2 * li + 1]);   // (vx, vy) interleaved: one 128-bit LDS read
             }
-            const double d0 = i - fx, d1 = j - fy;
-            const double w = wx[i] * wy[j];
-            v[0] += w * nvx;
-            v[1] += w * nvy;
-            const double s = s4 * w;
-            B[0] += s * (nvx * d0);
-            B[1] += s * (nvx * d1);
-            B[2] += s * (nvy * d0);
-            B[3] += s * (nvy * d1);
-        }
+    } else {
+#pragma unroll 1
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const long long gi = (long long)(bxc + i) * c.gridY + by + j;
+                gather(i, j, g[gpitch + gi], g[2 * gpitch + gi]);
+            }
+    }
     x += v[0] * c.dt;
     y += v[1] * c.dt;
     const double A[4] = {1.0 + c.dt * B[0], c.dt * B[1], c.dt * B[2], 1.0 + c.dt * B[3]};

@@ -139,30 +139,36 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
     v[0] = v[1] = 0;
     C[0] = C[1] = C[2] = C[3] = 0;
+    auto gather = [&](int i, int j, float nvx, float nvy) {
+        const float d0 = (float)i - fx, d1 = (float)j - fy;
+        const float w = wx[i] * wy[j];
+        v[0] += w * nvx;
+        v[1] += w * nvy;
+        const float s = cf.s4 * w;
+        C[0] += s * (nvx * d0);
+        C[1] += s * (nvx * d1);
+        C[2] += s * (nvy * d0);
+        C[3] += s * (nvy * d1);
+    };
+    // separate LDS and global loops, so that the LDS gather does not wait for
+    // the global loads in flight (the prefetch of the next point)
+    if (inwin) {
 #pragma unroll 1
-    for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            float nvx, nvy;
-            if (inwin) {
+            for (int j = 0; j < 3; j++) {
                 const int li = (lx + i) * win_h + ly + j;
-                nvx = vw[2 * li];           // (vx, vy) interleaved: one 64-bit LDS read
-                nvy = vw[2 * li + 1];
-            } else {
-                const long long gi = (long long)(bxc + i) * c.gridY + by + j;
-                nvx = g[kLinePxHi * gp + gi];
-                nvy = g[kLinePyHi * gp + gi];
+                gather(i, j, vw[2 * li], vw[2 * li + 1]);   // (vx, vy) interleaved: one 64-bit LDS read
             }
-            const float d0 = (float)i - fx, d1 = (float)j - fy;
-            const float w = wx[i] * wy[j];
-            v[0] += w * nvx;
-            v[1] += w * nvy;
-            const float s = cf.s4 * w;
-            C[0] += s * (nvx * d0);
-            C[1] += s * (nvx * d1);
-            C[2] += s * (nvy * d0);
-            C[3] += s * (nvy * d1);
-        }
+    } else {
+#pragma unroll 1
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const long long gi = (long long)(bxc + i) * c.gridY + by + j;
+                gather(i, j, g[kLinePxHi * gp + gi], g[kLinePyHi * gp + gi]);
+            }
+    }
     // x += v dt in fixed point, rounded stochastically (unbiased)
     xq += (unsigned)__float2int_rd(v[0] * cf.dq_scale + unit_hash(point, 2 * step));
     yq += (unsigned)__float2int_rd(v[1] * cf.dq_scale + unit_hash(point, 2 * step + 1));
