@@ -27,10 +27,13 @@ namespace fp64 {
 
 // ---------------------------------------------------------------------------
 // Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
-// Register budget capped at 3 waves per SIMD (<= 168 VGPRs, no spills).
+// 2 waves per SIMD (up to 256 VGPRs; the fused kernel needs about 204): at 3
+// waves it spilled about 20 VGPRs in the point loop, and every spill reload
+// waited for the prefetched loads of the next point (204 vs 230 us per
+// launch on the 4M-point sheet).
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_tiles(StepArgs a, int rbuf, int wbuf,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_tiles(StepArgs a, int rbuf, int wbuf,
                                                                                         int record_mode)
 {
     constexpr bool DO_G2P = (MODE & kTileG2P) != 0;
