@@ -994,7 +994,7 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     // are tried at two shapes only; the small tiles are for sparse scenes.
     static const int cand[][4] = {{4, 16, 1, 0}, {8, 8, 1, 0}, {4, 32, 1, 0}, {4, 64, 1, 0}, {8, 16, 1, 0},
                                   {8, 32, 1, 0}, {16, 16, 1, 0}, {4, 32, 0, 0}, {8, 32, 0, 0}, {4, 32, 2, 0},
-                                  {8, 32, 2, 0}};
+                                  {8, 32, 2, 0}, {16, 32, 1, 0}, {8, 64, 1, 0}, {32, 16, 1, 0}};
     const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
     // backup: points into the sort buffer, clocks and forces on the host
