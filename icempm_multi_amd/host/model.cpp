@@ -121,10 +121,12 @@ bool icy::Model::Step()
     }
 
     if (rebalance) {
+        if (verbose) std::printf("squeezing and sorting HSSOA\n");
         gpu.hssoa.RemoveDisabledAndSort(prms.cellsize_inv, prms.GridY);
         gpu.split_hssoa_into_partitions();
         gpu.transfer_ponts_to_device();
         SyncTopologyRequired = true;
+        if (verbose) std::printf("rebalancing done\n");
     }
     accessing_point_data.unlock();
     // with no completion callback nobody else would release the frame

@@ -82,6 +82,28 @@ def test_model_frames_fp64_match_oracle(tmp_path, mode):
     assert len(timings) == 2 and float(timings[-1].split(",")[-1]) > 0
 
 
+def test_model_rebalance_path_matches_oracle(tmp_path):
+    """ExtraSpaceForIncomingPoints = 0.1 leaves less free point space than
+    RebalanceThresholdFreeSpaceRemaining (0.15), so every frame ends with the
+    rebalance of model.cpp:102-110: host RemoveDisabledAndSort, re-split and
+    re-upload.  The run must still follow the oracle."""
+    d = np.load(os.path.join(H.GOLDEN, "ref_2d_70k.npz"))
+    H.write_flat_raw(tmp_path / "in.bin", d["x"], d["y"], d["llGrainIDs"], d["volume"][0])
+    H.run("--convert-raw", tmp_path / "in.bin", tmp_path / "pts.h5")
+    cfg = H.p1_config(str(tmp_path), ExtraSpaceForIncomingPoints=0.1)
+    P, _ = im.params_parse_json(open(cfg).read())
+    h, n = scenes.load_raw_points(P, d["x"], d["y"], d["llGrainIDs"], float(d["volume"][0]))
+    out_s = H.run(cfg, "--points", tmp_path / "pts.h5", "--frames", 2, "--log-dir", "",
+                  "--dump-soa", tmp_path / "out.bin", cwd=tmp_path)
+    assert out_s.count("rebalancing done") == 2
+    out, size = H.read_soa_dump(tmp_path / "out.bin")
+    assert size == n
+    ref, _ = oracle_frames(P, h, n, 2)
+    # the host sort after frame 1 orders the points as the oracle's sort does
+    O.sort(P, ref, n)
+    assert_close(out, ref, n, 1e-8, "rebalance")
+
+
 def test_model_frames_mixed_accuracy(tmp_path):
     cfg, P, h, n = inputs(tmp_path)
     H.run(cfg, "--points", tmp_path / "pts.h5", "--frames", 2, "--precision", "mixed", "--log-dir", "",
