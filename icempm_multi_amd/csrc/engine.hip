@@ -272,6 +272,9 @@ int Engine::commit()
         off += round_up(scenes_[i].cap_pts, 256);
     }
     const long long space = off;
+    // the kernels address a point line with a 32-bit byte offset (index * 8)
+    if (round_up(space, 256) >= kMaxPointSpace)
+        return fail(ICEMPM_ERR_CAPACITY, "point capacity of all scenes exceeds 2^29 - 256 points");
     bool points_layout_same = pts_[0] && space == n_pts_space_;
     for (int i = 0; i < nscenes() && points_layout_same; i++) points_layout_same = offs[i] == scenes_[i].pt_off;
     if (!points_layout_same) {

@@ -128,7 +128,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         const int n_it = a.lane_spread == 2 ? 2 * ((total + 2 * kBlock - 1) / (2 * kBlock))
                                             : (total + kBlock - 1) / kBlock;
         // point k of the tile -> SoA index; false past the end
-        auto locate = [&](int it, long long &pt) -> bool {
+        auto locate = [&](int it, unsigned &pt) -> bool {
             int k;
             if (a.lane_spread == 2)
                 k = (it >> 1) * (2 * kBlock) + wave * 128 + ((lane & 31) << 2) + (lane >> 5) + 2 * (it & 1);
@@ -143,7 +143,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             return true;
         };
         // the state of the next point is loaded while the current one is computed
-        long long pt_n = 0;
+        unsigned pt_n = 0;   // SoA index (points per launch < 2^29)
         unsigned ud_n = 0, xq_n = 0, yq_n = 0;
         float D_n[4] = {0, 0, 0, 0}, Jp_n = 0;
         bool ok_n = n_it > 0 && locate(0, pt_n);
@@ -154,7 +154,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 #pragma unroll 1
         for (int it = 0; it < n_it; it++) {
             const bool ok = ok_n;
-            const long long pt = pt_n;
+            const unsigned pt = pt_n;
             unsigned ud = ud_n, xq = xq_n, yq = yq_n;
             float D[4] = {D_n[0], D_n[1], D_n[2], D_n[3]}, Jp = Jp_n, v[2], C[4];
             ok_n = it + 1 < n_it && locate(it + 1, pt_n);

@@ -17,6 +17,9 @@ constexpr int kLineUtility = 0, kLineP = 1, kLineQ = 2, kLineJp = 3, kLinePosX =
               kLineFe00 = 8, kLineBp00 = 12;
 constexpr int64_t kCrushed = 0x10000, kDisabled = 0x20000, kLiquid = 0x40000;
 constexpr int kMaxIndenterSub = 1024;
+// Point lines are addressed by a 32-bit byte offset from the line base
+// (index * 8 in the FP64 form), so the point space of a launch stays below 2^29.
+constexpr long long kMaxPointSpace = 1LL << 29;
 // Grid nodes are tracked in segments of 64 consecutive nodes (one wave): a
 // per-segment flag says that a buffer may hold mass there, so the grid update
 // reads and clears only flagged segments.

@@ -289,3 +289,15 @@ def test_mass_conservation_sheet_fused():
     g = part.get_grid(0, nn)
     assert abs(g[0].sum() - n * P.ParticleMass) <= 1e-12 * n * P.ParticleMass
     assert part.error_code == 0
+
+
+def test_point_capacity_limit_is_enforced():
+    """Point lines are addressed with 32-bit byte offsets: a point space of
+    2^29 or more is refused before anything is allocated or launched."""
+    P, h, n = small_block(seed=0x1CE0800)
+    part = im.Partition(0, 0, im.PRECISION_FP64)
+    with pytest.raises(im.IcempmError):
+        part.allocate(1 << 29, P.GridXTotal)
+        part.update_constants(P)
+        part.transfer_points_from_soa_to_device(h, 0, n)
+    part.close()
