@@ -306,6 +306,8 @@ int Engine::commit()
         foff[i] = forces;
         forces += 2LL * P.n_indenter_subdivisions;
     }
+    // cell keys, bin counts and cell starts are 32-bit
+    if (cells >= (1LL << 31) - 1) return fail(ICEMPM_ERR_CAPACITY, "grid cells of all scenes exceed 2^31");
     const size_t buf_bytes = (size_t)gelems * eb;
     HIPCHK(hipMalloc(&grid_mem_, 2 * buf_bytes));
     HIPCHK(hipMemsetAsync(grid_mem_, 0, 2 * buf_bytes, stream_));
