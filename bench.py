@@ -31,15 +31,17 @@ METRIC = "M particle-steps/s + HBM GB/s fraction, mixed vs FP64 at ≤1e-4 rel p
 
 # Algorithmic HBM bytes per point and per touched grid node, per launch.
 # FP64 fused G2P+P2G: reads utility, Jp_inv, x, Fe (8+8+16+32 = 64 B) and writes
-# x, v, Fe, C, Jp_inv, utility (16+16+32+32+8+8 = 112 B) per point; per node it
-# reads the velocity (16 B) and accumulates mass and momentum (24 B).  The mixed
-# form keeps 4-byte point fields (32 + 56 B); its node velocity is 8 B and the
-# compensated (hi, lo) mass/momentum pairs 24 B.  A whole substep adds the grid
-# update (per node: read 3 values, write 2, clear the other buffer).
+# x and Fe (16+32 = 48 B) per point: v and C pass to P2G in registers and are
+# stored only by the plain G2P launch that ends a call, Jp_inv and the utility
+# word only for crushed or liquid points.  Per node it reads the velocity (16 B)
+# and accumulates mass and momentum (24 B).  The mixed form keeps 4-byte point
+# fields (32 + 24 B); its node velocity is 8 B and the compensated (hi, lo)
+# mass/momentum pairs 24 B.  A whole substep adds the grid update (per node:
+# read 3 values, write 2, clear the other buffer).
 BYTES = {
-    "fp64": {"fused_point": 64 + 112, "fused_node": 16 + 24, "p2g_point": 104, "g2p_point": 64 + 112,
+    "fp64": {"fused_point": 64 + 48, "fused_node": 16 + 24, "p2g_point": 104, "g2p_point": 64 + 112,
              "step_point": 280, "step_node": 80},
-    "mixed": {"fused_point": 32 + 56, "fused_node": 8 + 24, "p2g_point": 52, "g2p_point": 32 + 56,
+    "mixed": {"fused_point": 32 + 24, "fused_node": 8 + 24, "p2g_point": 52, "g2p_point": 32 + 56,
               "step_point": 140, "step_node": 64},
 }
 

@@ -189,7 +189,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                     err = max(err, e);
                     continue;
                 }
-                store_state(R, pt, x, y, v, F, B, Jp, ud, record, pp, qq);
+                if (DO_P2G) store_state_fused(R, pt, x, y, F, Jp, ud, record, pp, qq);
+                else store_state(R, pt, x, y, v, F, B, Jp, ud, record, pp, qq);
             } else {
                 v[0] = R.at(kLineVelX, pt);
                 v[1] = R.at(kLineVelX + 1, pt);

@@ -172,7 +172,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                     err = max(err, e);
                     continue;
                 }
-                store_state(R, pt, xq, yq, v, D, C, Jp, ud, record, pp, qq);
+                if (DO_P2G) store_state_fused(R, pt, xq, yq, D, Jp, ud, record, pp, qq);
+                else store_state(R, pt, xq, yq, v, D, C, Jp, ud, record, pp, qq);
             } else {
                 v[0] = R.f(kLineVelX, pt);
                 v[1] = R.f(kLineVelX + 1, pt);

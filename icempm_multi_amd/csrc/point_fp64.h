@@ -186,5 +186,29 @@ __device__ __forceinline__ void store_state(const PtRef &R, long long pt, double
     }
 }
 
+// State stores of the fused G2P(n)+P2G(n+1) launch.  v and C go to P2G in
+// registers, and the next launch's G2P computes them afresh from the grid:
+// they are read from memory only by a call's first P2G and by the download,
+// both of which follow a plain G2P launch that stores them.  So the fused
+// launch stores x and F, Jp_inv and the utility word only for crushed and
+// liquid points (the only ones whose Jp_inv or flags change), and P, Q on
+// recording substeps.
+__device__ __forceinline__ void store_state_fused(const PtRef &R, long long pt, double x, double y, const double F[4],
+                                                  double Jp, long long ud, bool record, double p, double q)
+{
+    R.at(kLinePosX, pt) = x;
+    R.at(kLinePosX + 1, pt) = y;
+#pragma unroll
+    for (int k = 0; k < 4; k++) R.at(kLineFe00 + k, pt) = F[k];
+    if (ud & (kLiquid | kCrushed)) {   // the only points whose Jp_inv or flags can change
+        R.at(kLineJp, pt) = Jp;
+        R.at(kLineUtility, pt) = as_d(ud);
+    }
+    if (record) {
+        R.at(kLineP, pt) = p;
+        R.at(kLineQ, pt) = q;
+    }
+}
+
 }  // namespace fp64
 }  // namespace icempm

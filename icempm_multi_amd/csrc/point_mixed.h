@@ -252,6 +252,26 @@ __device__ __forceinline__ void store_state(const PtRef &R, long long pt, unsign
     }
 }
 
+// State stores of the fused launch (see the FP64 form): x, y and D; Jp_inv
+// and the utility word of crushed and liquid points; P, Q on recording substeps.
+__device__ __forceinline__ void store_state_fused(const PtRef &R, long long pt, unsigned xq, unsigned yq,
+                                                  const float D[4], float Jp, unsigned ud, bool record, float p,
+                                                  float q)
+{
+    R.u(kLinePosX, pt) = xq;
+    R.u(kLinePosX + 1, pt) = yq;
+#pragma unroll
+    for (int k = 0; k < 4; k++) R.setf(kLineFe00 + k, pt, D[k]);
+    if (ud & (unsigned)(kLiquid | kCrushed)) {
+        R.setf(kLineJp, pt, Jp);
+        R.u(kLineUtility, pt) = ud;
+    }
+    if (record) {
+        R.setf(kLineP, pt, p);
+        R.setf(kLineQ, pt, q);
+    }
+}
+
 __device__ __forceinline__ int xcd_block()
 {
     const int G = gridDim.x;
