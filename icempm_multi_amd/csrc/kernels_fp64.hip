@@ -32,7 +32,11 @@ namespace fp64 {
 // waited for the prefetched loads of the next point (204 vs 230 us per
 // launch on the 4M-point sheet).
 // ---------------------------------------------------------------------------
-template <int MODE>
+//
+// WHC is the window height (ty + 2 + 2*margin) when fixed at compile time for
+// the common tile shapes (0: read from the arguments).  A compile-time height
+// turns the stencil's LDS offsets into instruction immediates.
+template <int MODE, int WHC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_tiles(StepArgs a, int rbuf, int wbuf,
                                                                                         int record_mode)
 {
@@ -42,10 +46,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
 
-    const int WN = a.win_w * a.win_h;
+    const int win_h = WHC > 0 ? WHC : a.win_h, win_w = a.win_w;
+    const int WN = win_w * win_h;
     lds_d *lds = (lds_d *)lds_raw;
-    lds_d *vw = lds;                       // 2*WN: (vx, vy) per window node
-    lds_d *am = lds + (DO_G2P ? 2 * WN : 0), *apx = am + WN, *apy = apx + WN;
+    lds_d *vw = lds;                            // 2*WN: (vx, vy) per window node
+    lds_d *acc = lds + (DO_G2P ? 2 * WN : 0);   // 3*WN: (m, px, py) per window node
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(double *)a.pts, a.pitch};
@@ -111,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
         // stage velocities / clear accumulators
         for (int i = tid; i < WN; i += kBlock) {
             if (DO_G2P) {
-                const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
+                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 double vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
                     const long long gi = (long long)col * gridY + row;
@@ -121,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                 vw[2 * i] = vx;
                 vw[2 * i + 1] = vy;
             }
-            if (DO_P2G) { am[i] = 0; apx[i] = 0; apy[i] = 0; }
+            if (DO_P2G) { acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0; }
         }
         __syncthreads();
         if (tid == 0) {
@@ -185,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
             if (DO_G2P) {
                 double pp, qq;
                 unsigned e;
-                if (!g2p_point(c, gr, gp, vw, wc0, wr0, a.win_w, a.win_h, x, y, v, B, F, Jp, ud, pp, qq, e)) {
+                if (!g2p_point(c, gr, gp, vw, wc0, wr0, win_w, win_h, x, y, v, B, F, Jp, ud, pp, qq, e)) {
                     err = max(err, e);
                     continue;
                 }
@@ -198,37 +203,52 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                 for (int m = 0; m < 4; m++) B[m] = R.at(kLineBp00 + m, pt);
             }
             if (DO_P2G) {
-                const unsigned e = p2g_point(c, x, y, v, B, F, Jp, ud,
-                    [&](int col, int row, double dm, double dpx, double dpy) {
-                        const int lx = col - wc0, ly = row - wr0;
-                        if (lx >= 0 && lx < a.win_w && ly >= 0 && ly < a.win_h) {
-                            const int li = lx * a.win_h + ly;
-                            lds_add(am + li, dm);
-                            lds_add(apx + li, dpx);
-                            lds_add(apy + li, dpy);
+                P2GStencil st;
+                const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, st);
+                if (e) {
+                    err = max(err, e);
+                    continue;
+                }
+                const int lx = st.bxc - wc0, ly = st.by - wr0;
+                if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
+                    lds_d *const p0 = acc + 3 * (lx * win_h + ly);
+                    st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
+                        lds_d *const p = p0 + 3 * (i * win_h + j);
+                        lds_add(p, dm);
+                        lds_add(p + 1, dpx);
+                        lds_add(p + 2, dpy);
+                    });
+                } else {
+                    // the stencil leaves the window (a point that drifted since the sort)
+                    st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
+                        if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
+                            lds_d *const p = acc + 3 * ((lx + i) * win_h + ly + j);
+                            lds_add(p, dm);
+                            lds_add(p + 1, dpx);
+                            lds_add(p + 2, dpy);
                         } else {
-                            const long long gi = (long long)col * gridY + row;
+                            const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
                             glb_add(gw + gi, dm);
                             glb_add(gw + gp + gi, dpx);
                             glb_add(gw + 2 * gp + gi, dpy);
                             dflag[gi >> kSegShift] = 1;
                         }
                     });
-                err = max(err, e);
+                }
             }
         }
         if (err) atomicMax(&a.err[ti.scene], err);
         if (DO_P2G) {
             __syncthreads();
             for (int i = tid; i < WN; i += kBlock) {
-                const double m = am[i];
+                const double m = acc[3 * i];
                 if (m == 0) continue;
-                const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
+                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
                 glb_add(gw + gi, m);
-                glb_add(gw + gp + gi, apx[i]);
-                glb_add(gw + 2 * gp + gi, apy[i]);
+                glb_add(gw + gp + gi, acc[3 * i + 1]);
+                glb_add(gw + 2 * gp + gi, acc[3 * i + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }
@@ -328,25 +348,31 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
     return n * sizeof(double);
 }
 
+// window heights compiled in: ty = 16, 32, 64 with a drift margin of 1
+template <int MODE>
+static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int record_mode, int n_blocks, size_t lds,
+                                  hipStream_t s)
+{
+    switch (a.win_h) {
+    case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
                         hipStream_t s)
 {
     if (a.tx > kMaxTX || n_blocks <= 0) return hipErrorInvalidValue;
     const size_t lds = tile_lds_bytes(a, mode);
     switch (mode) {
-    case kTileG2P:
-        hipLaunchKernelGGL(k_tiles<kTileG2P>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
-        break;
-    case kTileP2G:
-        hipLaunchKernelGGL(k_tiles<kTileP2G>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
-        break;
-    case kTileFused:
-        hipLaunchKernelGGL(k_tiles<kTileFused>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case kTileG2P: return launch_tiles_wh<kTileG2P>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    case kTileP2G: return launch_tiles_wh<kTileP2G>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    case kTileFused: return launch_tiles_wh<kTileFused>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s)

@@ -16,8 +16,9 @@
 namespace icempm {
 namespace mixed {
 
-template <int MODE>
-// 4 waves per SIMD (<= 128 VGPRs): a few spills, but more latency hidden than at 3
+// 4 waves per SIMD (<= 128 VGPRs).  WHC: the window height when fixed at
+// compile time (0: read from the arguments), as in the FP64 form.
+template <int MODE, int WHC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 {
@@ -27,10 +28,11 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
 
-    const int WN = a.win_w * a.win_h;
-    // accumulators (FP64) first, then the FP32 velocity window
-    lds_d *am = (lds_d *)lds_raw, *apx = am + WN, *apy = apx + WN;
-    lds_f *vw = (lds_f *)(am + (DO_P2G ? 3 * WN : 0));   // 2*WN: (vx, vy) per window node
+    const int win_h = WHC > 0 ? WHC : a.win_h, win_w = a.win_w;
+    const int WN = win_w * win_h;
+    // accumulators (FP64, (m, px, py) per window node) first, then the FP32 velocity window
+    lds_d *acc = (lds_d *)lds_raw;
+    lds_f *vw = (lds_f *)(acc + (DO_P2G ? 3 * WN : 0));   // 2*WN: (vx, vy) per window node
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(unsigned *)a.pts, a.pitch};
@@ -95,7 +97,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         }
         for (int i = tid; i < WN; i += kBlock) {
             if (DO_G2P) {
-                const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
+                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 float vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
                     const long long gi = (long long)col * gridY + row;
@@ -105,7 +107,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 vw[2 * i] = vx;
                 vw[2 * i + 1] = vy;
             }
-            if (DO_P2G) { am[i] = 0; apx[i] = 0; apy[i] = 0; }
+            if (DO_P2G) { acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0; }
         }
         __syncthreads();
         if (tid == 0) {
@@ -167,7 +169,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             if (DO_G2P) {
                 float pp, qq;
                 unsigned e;
-                if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, a.win_w, a.win_h, xq, yq, v, C, D, Jp, ud, pp, qq,
+                if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, win_w, win_h, xq, yq, v, C, D, Jp, ud, pp, qq,
                                e, (unsigned)pt, (unsigned)step)) {
                     err = max(err, e);
                     continue;
@@ -181,37 +183,52 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 for (int m = 0; m < 4; m++) C[m] = R.f(kLineBp00 + m, pt);
             }
             if (DO_P2G) {
-                const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jp, ud,
-                    [&](int col, int row, float dm, float dpx, float dpy) {
-                        const int lx = col - wc0, ly = row - wr0;
-                        if (lx >= 0 && lx < a.win_w && ly >= 0 && ly < a.win_h) {
-                            const int li = lx * a.win_h + ly;
-                            lds_add(am + li, (double)dm);
-                            lds_add(apx + li, (double)dpx);
-                            lds_add(apy + li, (double)dpy);
+                P2GStencil st;
+                const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, st);
+                if (e) {
+                    err = max(err, e);
+                    continue;
+                }
+                const int lx = st.bxc - wc0, ly = st.by - wr0;
+                if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
+                    lds_d *const p0 = acc + 3 * (lx * win_h + ly);
+                    st.nodes(cf, [&](int i, int j, float dm, float dpx, float dpy) {
+                        lds_d *const p = p0 + 3 * (i * win_h + j);
+                        lds_add(p, (double)dm);
+                        lds_add(p + 1, (double)dpx);
+                        lds_add(p + 2, (double)dpy);
+                    });
+                } else {
+                    // the stencil leaves the window (a point that drifted since the sort)
+                    st.nodes(cf, [&](int i, int j, float dm, float dpx, float dpy) {
+                        if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
+                            lds_d *const p = acc + 3 * ((lx + i) * win_h + ly + j);
+                            lds_add(p, (double)dm);
+                            lds_add(p + 1, (double)dpx);
+                            lds_add(p + 2, (double)dpy);
                         } else {
-                            const long long gi = (long long)col * gridY + row;
+                            const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
                             twosum_add(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm, 0.f);
                             twosum_add(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx, 0.f);
                             twosum_add(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy, 0.f);
                             dflag[gi >> kSegShift] = 1;
                         }
                     });
-                err = max(err, e);
+                }
             }
         }
         if (err) atomicMax(&a.err[ti.scene], err);
         if (DO_P2G) {
             __syncthreads();
             for (int i = tid; i < WN; i += kBlock) {
-                const double m = am[i];
+                const double m = acc[3 * i];
                 if (m == 0) continue;
-                const int col = wc0 + i / a.win_h, row = wr0 + i % a.win_h;
+                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
                 twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, m);
-                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, apx[i]);
-                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, apy[i]);
+                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, acc[3 * i + 1]);
+                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, acc[3 * i + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }
@@ -308,25 +325,31 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
     return n;
 }
 
+// window heights compiled in: ty = 16, 32, 64 with a drift margin of 1
+template <int MODE>
+static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int record_mode, int n_blocks, size_t lds,
+                                  hipStream_t s)
+{
+    switch (a.win_h) {
+    case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
                         hipStream_t s)
 {
     if (a.tx > kMaxTX || n_blocks <= 0) return hipErrorInvalidValue;
     const size_t lds = tile_lds_bytes(a, mode);
     switch (mode) {
-    case kTileG2P:
-        hipLaunchKernelGGL(k_tiles<kTileG2P>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
-        break;
-    case kTileP2G:
-        hipLaunchKernelGGL(k_tiles<kTileP2G>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
-        break;
-    case kTileFused:
-        hipLaunchKernelGGL(k_tiles<kTileFused>, dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case kTileG2P: return launch_tiles_wh<kTileG2P>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    case kTileP2G: return launch_tiles_wh<kTileP2G>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    case kTileFused: return launch_tiles_wh<kTileFused>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s)

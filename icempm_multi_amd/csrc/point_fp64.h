@@ -116,42 +116,65 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     return true;
 }
 
-// P2G contributions of one point (kernels.cuh:59-121).  Calls sink(col, row,
-// dm, dpx, dpy) for the 9 stencil nodes; returns the error byte (0 = ok).
+// P2G stencil of one point (kernels.cuh:59-121): base node, weights and the
+// affine momentum terms.  nodes() calls f(i, j, dm, dpx, dpy) for the 9
+// stencil nodes (base + (i, j)).
+struct P2GStencil {
+    int bxc, by;
+    double fx, fy, wx[3], wy[3], S[4], mv0, mv1, mass;
+
+    template <class F>
+    __device__ __forceinline__ void nodes(const SceneConsts &c, F &&f) const
+    {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double W = wx[i] * wy[j];
+                const double d0 = (i - fx) * c.h, d1 = (j - fy) * c.h;
+                const double t0 = mv0 + (S[0] * d0 + S[1] * d1);
+                const double t1 = mv1 + (S[2] * d0 + S[3] * d1);
+                f(i, j, W * mass, W * t0, W * t1);
+            }
+    }
+};
+
+// Stress, base node and weights of a point; returns the error byte (0 = ok).
+__device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, double x, double y, const double v[2],
+                                                const double B[4], const double F[4], double Jp, long long ud,
+                                                P2GStencil &st)
+{
+    const bool liquid = (ud & kLiquid) != 0;
+    st.mass = liquid ? c.wmass : c.pmass;
+    double tau[4];
+    if (liquid) water(c, Jp, tau);
+    else kirchhoff(c, F, tau);
+#pragma unroll
+    for (int k = 0; k < 4; k++) st.S[k] = st.mass * B[k] - c.dt_vol_Dpinv * tau[k];
+    int bx;
+    base_fx(c, x, bx, st.fx);
+    base_fx(c, y, st.by, st.fy);
+    st.bxc = bx - c.xoff + c.halo;
+    const unsigned e = stencil_error(c, st.bxc, st.by);
+    if (e) return e;
+    weights(st.fx, st.wx);
+    weights(st.fy, st.wy);
+    st.mv0 = v[0] * st.mass;
+    st.mv1 = v[1] * st.mass;
+    return 0;
+}
+
+// P2G contributions of one point.  Calls sink(col, row, dm, dpx, dpy) for
+// the 9 stencil nodes; returns the error byte (0 = ok).
 template <class Sink>
 __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, double x, double y, const double v[2],
                                               const double B[4], const double F[4], double Jp, long long ud,
                                               Sink &&sink)
 {
-    const bool liquid = (ud & kLiquid) != 0;
-    const double mass = liquid ? c.wmass : c.pmass;
-    double tau[4];
-    if (liquid) water(c, Jp, tau);
-    else kirchhoff(c, F, tau);
-    double S[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) S[k] = mass * B[k] - c.dt_vol_Dpinv * tau[k];
-    int bx, by;
-    double fx, fy;
-    base_fx(c, x, bx, fx);
-    base_fx(c, y, by, fy);
-    const int bxc = bx - c.xoff + c.halo;
-    const unsigned e = stencil_error(c, bxc, by);
+    P2GStencil st;
+    const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, st);
     if (e) return e;
-    double wx[3], wy[3];
-    weights(fx, wx);
-    weights(fy, wy);
-    const double mv0 = v[0] * mass, mv1 = v[1] * mass;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const double W = wx[i] * wy[j];
-            const double d0 = (i - fx) * c.h, d1 = (j - fy) * c.h;
-            const double t0 = mv0 + (S[0] * d0 + S[1] * d1);
-            const double t1 = mv1 + (S[2] * d0 + S[3] * d1);
-            sink(bxc + i, by + j, W * mass, W * t0, W * t1);
-        }
+    st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
     return 0;
 }
 

@@ -183,41 +183,63 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     return true;
 }
 
+// P2G stencil of one point: base node, weights and the affine momentum
+// terms.  nodes() calls f(i, j, dm, dpx, dpy) for the 9 stencil nodes.
+struct P2GStencil {
+    int bxc, by;
+    float fx, fy, wx[3], wy[3], S[4], mv0, mv1, mass;
+
+    template <class F>
+    __device__ __forceinline__ void nodes(const ConstsF &cf, F &&f) const
+    {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const float W = wx[i] * wy[j];
+                const float d0 = ((float)i - fx) * cf.h, d1 = ((float)j - fy) * cf.h;
+                const float t0 = mv0 + (S[0] * d0 + S[1] * d1);
+                const float t1 = mv1 + (S[2] * d0 + S[3] * d1);
+                f(i, j, W * mass, W * t0, W * t1);
+            }
+    }
+};
+
+// Stress, base node and weights of a point; returns the error byte (0 = ok).
+__device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
+                                                const float v[2], const float C[4], const float D[4], float Jp,
+                                                unsigned ud, P2GStencil &st)
+{
+    const bool liquid = (ud & (unsigned)kLiquid) != 0;
+    st.mass = liquid ? cf.wmass : cf.pmass;
+    float tau[4];
+    if (liquid) water(cf, Jp, tau);
+    else kirchhoff(cf, D, tau);
+#pragma unroll
+    for (int k = 0; k < 4; k++) st.S[k] = st.mass * C[k] - cf.dt_vol_Dpinv * tau[k];
+    int bx;
+    base_fx_fixed(xq, cf.fb, cf.origin, bx, st.fx);
+    base_fx_fixed(yq, cf.fb, cf.origin_y, st.by, st.fy);
+    st.bxc = bx - c.xoff + c.halo;
+    const unsigned e = stencil_error(c, st.bxc, st.by);
+    if (e) return e;
+    weights(st.fx, st.wx);
+    weights(st.fy, st.wy);
+    st.mv0 = v[0] * st.mass;
+    st.mv1 = v[1] * st.mass;
+    return 0;
+}
+
 // P2G contributions of one point; calls sink(col, row, dm, dpx, dpy).
 template <class Sink>
 __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
                                               const float v[2], const float C[4], const float D[4], float Jp,
                                               unsigned ud, Sink &&sink)
 {
-    const bool liquid = (ud & (unsigned)kLiquid) != 0;
-    const float mass = liquid ? cf.wmass : cf.pmass;
-    float tau[4];
-    if (liquid) water(cf, Jp, tau);
-    else kirchhoff(cf, D, tau);
-    float S[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) S[k] = mass * C[k] - cf.dt_vol_Dpinv * tau[k];
-    int bx, by;
-    float fx, fy;
-    base_fx_fixed(xq, cf.fb, cf.origin, bx, fx);
-    base_fx_fixed(yq, cf.fb, cf.origin_y, by, fy);
-    const int bxc = bx - c.xoff + c.halo;
-    const unsigned e = stencil_error(c, bxc, by);
+    P2GStencil st;
+    const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, st);
     if (e) return e;
-    float wx[3], wy[3];
-    weights(fx, wx);
-    weights(fy, wy);
-    const float mv0 = v[0] * mass, mv1 = v[1] * mass;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const float W = wx[i] * wy[j];
-            const float d0 = ((float)i - fx) * cf.h, d1 = ((float)j - fy) * cf.h;
-            const float t0 = mv0 + (S[0] * d0 + S[1] * d1);
-            const float t1 = mv1 + (S[2] * d0 + S[3] * d1);
-            sink(bxc + i, by + j, W * mass, W * t0, W * t1);
-        }
+    st.nodes(cf, [&](int i, int j, float dm, float dpx, float dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
     return 0;
 }
 
