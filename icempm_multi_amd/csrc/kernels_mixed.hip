@@ -192,25 +192,25 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 const int lx = st.bxc - wc0, ly = st.by - wr0;
                 if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
                     lds_d *const p0 = acc + 3 * (lx * win_h + ly);
-                    st.nodes(cf, [&](int i, int j, float dm, float dpx, float dpy) {
+                    st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
                         lds_d *const p = p0 + 3 * (i * win_h + j);
-                        lds_add(p, (double)dm);
-                        lds_add(p + 1, (double)dpx);
-                        lds_add(p + 2, (double)dpy);
+                        lds_add(p, dm);
+                        lds_add(p + 1, dpx);
+                        lds_add(p + 2, dpy);
                     });
                 } else {
                     // the stencil leaves the window (a point that drifted since the sort)
-                    st.nodes(cf, [&](int i, int j, float dm, float dpx, float dpy) {
+                    st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
                         if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
                             lds_d *const p = acc + 3 * ((lx + i) * win_h + ly + j);
-                            lds_add(p, (double)dm);
-                            lds_add(p + 1, (double)dpx);
-                            lds_add(p + 2, (double)dpy);
+                            lds_add(p, dm);
+                            lds_add(p + 1, dpx);
+                            lds_add(p + 2, dpy);
                         } else {
                             const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
-                            twosum_add(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm, 0.f);
-                            twosum_add(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx, 0.f);
-                            twosum_add(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy, 0.f);
+                            twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
+                            twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
+                            twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy);
                             dflag[gi >> kSegShift] = 1;
                         }
                     });

@@ -29,11 +29,11 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
     glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
     const long long gp = S.gpitch;
     const int gridY = c.gridY;
-    const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jp, ud, [&](int col, int row, float dm, float dpx, float dpy) {
+    const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jp, ud, [&](int col, int row, double dm, double dpx, double dpy) {
         const long long gi = (long long)col * gridY + row;
-        twosum_add(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm, 0.f);
-        twosum_add(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx, 0.f);
-        twosum_add(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy, 0.f);
+        twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
+        twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
+        twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy);
         mark_dirty(a, S, wbuf, gi);
     });
     if (e) atomicMax(&a.err[scene], e);

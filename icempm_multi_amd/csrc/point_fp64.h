@@ -73,41 +73,63 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     weights(fy, wy);
     const int lx = bxc - wc0, ly = by - wr0;
     const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
-    v[0] = v[1] = 0;
-    B[0] = B[1] = B[2] = B[3] = 0;
-    const double s4 = 4. * c.hinv;
-    auto gather = [&](int i, int j, double nvx, double nvy) {
-        const double d0 = i - fx, d1 = j - fy;
-        const double w = wx[i] * wy[j];
-        v[0] += w * nvx;
-        v[1] += w * nvy;
-        const double s = s4 * w;
-        B[0] += s * (nvx * d0);
-        B[1] += s * (nvx * d1);
-        B[2] += s * (nvy * d0);
-        B[3] += s * (nvy * d1);
+    // The weight of node (i, j) is wx[i] * wy[j], so the sums separate: per
+    // column i, a = sum_j wy[j] v_ij and b = sum_j wy[j] (j - fy) v_ij, then
+    // v = sum_i wx[i] a, C = (4/h) sum_i (wx[i] (i - fx) a, wx[i] b)
+    // (kernels.cuh:357-372 in factored order).
+    const double wyd[3] = {wy[0] * (0 - fy), wy[1] * (1 - fy), wy[2] * (2 - fy)};
+    double vx = 0, vy = 0, c00 = 0, c01 = 0, c10 = 0, c11 = 0;
+    auto column = [&](int i, double ax, double ay, double bx, double by) {
+        const double w = wx[i], wd = wx[i] * (i - fx);
+        vx += w * ax;
+        vy += w * ay;
+        c00 += wd * ax;
+        c01 += w * bx;
+        c10 += wd * ay;
+        c11 += w * by;
     };
     // Two separate loops: if the LDS and the global reads fed one loop, the
     // merged value would make every gather wait for all outstanding global
-    // loads, including the prefetch of the next point.  The gather is left
-    // rolled in i: fully unrolled it keeps 18 node values live.
+    // loads, including the prefetch of the next point.  The columns are left
+    // rolled: fully unrolled they keep 18 node values live.
     if (inwin) {
 #pragma unroll 1
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) {
+            double ax = 0, ay = 0, bx = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int li = (lx + i) * win_h + ly + j;
-                gather(i, j, vw[2 * li], vw[2 * li + 1]);   // (vx, vy) interleaved: one 128-bit LDS read
+                const double nvx = vw[2 * li], nvy = vw[2 * li + 1];   // one 128-bit LDS read
+                ax += wy[j] * nvx;
+                ay += wy[j] * nvy;
+                bx += wyd[j] * nvx;
+                by_ += wyd[j] * nvy;
             }
+            column(i, ax, ay, bx, by_);
+        }
     } else {
 #pragma unroll 1
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) {
+            double ax = 0, ay = 0, bx = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const long long gi = (long long)(bxc + i) * c.gridY + by + j;
-                gather(i, j, g[gpitch + gi], g[2 * gpitch + gi]);
+                const double nvx = g[gpitch + gi], nvy = g[2 * gpitch + gi];
+                ax += wy[j] * nvx;
+                ay += wy[j] * nvy;
+                bx += wyd[j] * nvx;
+                by_ += wyd[j] * nvy;
             }
+            column(i, ax, ay, bx, by_);
+        }
     }
+    const double s4 = 4. * c.hinv;
+    v[0] = vx;
+    v[1] = vy;
+    B[0] = s4 * c00;
+    B[1] = s4 * c01;
+    B[2] = s4 * c10;
+    B[3] = s4 * c11;
     x += v[0] * c.dt;
     y += v[1] * c.dt;
     const double A[4] = {1.0 + c.dt * B[0], c.dt * B[1], c.dt * B[2], 1.0 + c.dt * B[3]};
@@ -123,19 +145,26 @@ struct P2GStencil {
     int bxc, by;
     double fx, fy, wx[3], wy[3], S[4], mv0, mv1, mass;
 
+    // Node (i, j) gets W = wx[i] wy[j] times (m, mv + S (d0_i, d1_j)), with
+    // d0_i = (i - fx) h and d1_j = (j - fy) h (kernels.cuh:105-121), formed as
+    // W m = (wx[i] m) wy[j] and W (mv0 + S00 d0_i + S01 d1_j)
+    // = (wx[i] (mv0 + S00 d0_i)) wy[j] + (wx[i] S01) (wy[j] d1_j).
     template <class F>
     __device__ __forceinline__ void nodes(const SceneConsts &c, F &&f) const
     {
+        double wyd[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) wyd[j] = wy[j] * ((j - fy) * c.h);
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const double W = wx[i] * wy[j];
-                const double d0 = (i - fx) * c.h, d1 = (j - fy) * c.h;
-                const double t0 = mv0 + (S[0] * d0 + S[1] * d1);
-                const double t1 = mv1 + (S[2] * d0 + S[3] * d1);
-                f(i, j, W * mass, W * t0, W * t1);
-            }
+        for (int i = 0; i < 3; i++) {
+            const double d0 = (i - fx) * c.h;
+            const double w = wx[i];
+            const double wm = w * mass;
+            const double px = w * (mv0 + S[0] * d0), qx = w * S[1];
+            const double py = w * (mv1 + S[2] * d0), qy = w * S[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) f(i, j, wm * wy[j], px * wy[j] + qx * wyd[j], py * wy[j] + qy * wyd[j]);
+        }
     }
 };
 

@@ -137,38 +137,59 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     weights(fy, wy);
     const int lx = bxc - wc0, ly = by - wr0;
     const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
-    v[0] = v[1] = 0;
-    C[0] = C[1] = C[2] = C[3] = 0;
-    auto gather = [&](int i, int j, float nvx, float nvy) {
-        const float d0 = (float)i - fx, d1 = (float)j - fy;
-        const float w = wx[i] * wy[j];
-        v[0] += w * nvx;
-        v[1] += w * nvy;
-        const float s = cf.s4 * w;
-        C[0] += s * (nvx * d0);
-        C[1] += s * (nvx * d1);
-        C[2] += s * (nvy * d0);
-        C[3] += s * (nvy * d1);
+    // factored sums, as in the FP64 form: per column i, a = sum_j wy[j] v_ij
+    // and b = sum_j wy[j] (j - fy) v_ij; v = sum_i wx[i] a and
+    // C = (4/h) sum_i (wx[i] (i - fx) a, wx[i] b)
+    const float wyd[3] = {wy[0] * (0.f - fy), wy[1] * (1.f - fy), wy[2] * (2.f - fy)};
+    float vx = 0, vy = 0, c00 = 0, c01 = 0, c10 = 0, c11 = 0;
+    auto column = [&](int i, float ax, float ay, float bx_, float by_) {
+        const float w = wx[i], wd = wx[i] * ((float)i - fx);
+        vx += w * ax;
+        vy += w * ay;
+        c00 += wd * ax;
+        c01 += w * bx_;
+        c10 += wd * ay;
+        c11 += w * by_;
     };
     // separate LDS and global loops, so that the LDS gather does not wait for
     // the global loads in flight (the prefetch of the next point)
     if (inwin) {
 #pragma unroll 1
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) {
+            float ax = 0, ay = 0, bx_ = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int li = (lx + i) * win_h + ly + j;
-                gather(i, j, vw[2 * li], vw[2 * li + 1]);   // (vx, vy) interleaved: one 64-bit LDS read
+                const float nvx = vw[2 * li], nvy = vw[2 * li + 1];   // (vx, vy) interleaved: one 64-bit LDS read
+                ax += wy[j] * nvx;
+                ay += wy[j] * nvy;
+                bx_ += wyd[j] * nvx;
+                by_ += wyd[j] * nvy;
             }
+            column(i, ax, ay, bx_, by_);
+        }
     } else {
 #pragma unroll 1
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) {
+            float ax = 0, ay = 0, bx_ = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const long long gi = (long long)(bxc + i) * c.gridY + by + j;
-                gather(i, j, g[kLinePxHi * gp + gi], g[kLinePyHi * gp + gi]);
+                const float nvx = g[kLinePxHi * gp + gi], nvy = g[kLinePyHi * gp + gi];
+                ax += wy[j] * nvx;
+                ay += wy[j] * nvy;
+                bx_ += wyd[j] * nvx;
+                by_ += wyd[j] * nvy;
             }
+            column(i, ax, ay, bx_, by_);
+        }
     }
+    v[0] = vx;
+    v[1] = vy;
+    C[0] = cf.s4 * c00;
+    C[1] = cf.s4 * c01;
+    C[2] = cf.s4 * c10;
+    C[3] = cf.s4 * c11;
     // x += v dt in fixed point, rounded stochastically (unbiased)
     xq += (unsigned)__float2int_rd(v[0] * cf.dq_scale + unit_hash(point, 2 * step));
     yq += (unsigned)__float2int_rd(v[1] * cf.dq_scale + unit_hash(point, 2 * step + 1));
@@ -184,7 +205,8 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
 }
 
 // P2G stencil of one point: base node, weights and the affine momentum
-// terms.  nodes() calls f(i, j, dm, dpx, dpy) for the 9 stencil nodes.
+// terms.  nodes() calls f(i, j, dm, dpx, dpy) for the 9 stencil nodes, in the
+// factored order of the FP64 form, widened to FP64 for the accumulators.
 struct P2GStencil {
     int bxc, by;
     float fx, fy, wx[3], wy[3], S[4], mv0, mv1, mass;
@@ -192,16 +214,20 @@ struct P2GStencil {
     template <class F>
     __device__ __forceinline__ void nodes(const ConstsF &cf, F &&f) const
     {
+        float wyd[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) wyd[j] = wy[j] * (((float)j - fy) * cf.h);
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const float W = wx[i] * wy[j];
-                const float d0 = ((float)i - fx) * cf.h, d1 = ((float)j - fy) * cf.h;
-                const float t0 = mv0 + (S[0] * d0 + S[1] * d1);
-                const float t1 = mv1 + (S[2] * d0 + S[3] * d1);
-                f(i, j, W * mass, W * t0, W * t1);
-            }
+        for (int i = 0; i < 3; i++) {
+            const float d0 = ((float)i - fx) * cf.h;
+            const float w = wx[i];
+            const float wm = w * mass;
+            const float px = w * (mv0 + S[0] * d0), qx = w * S[1];
+            const float py = w * (mv1 + S[2] * d0), qy = w * S[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                f(i, j, (double)(wm * wy[j]), (double)(px * wy[j] + qx * wyd[j]), (double)(py * wy[j] + qy * wyd[j]));
+        }
     }
 };
 
@@ -230,7 +256,7 @@ __device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, const Cons
     return 0;
 }
 
-// P2G contributions of one point; calls sink(col, row, dm, dpx, dpy).
+// P2G contributions of one point; calls sink(col, row, dm, dpx, dpy) (FP64 terms).
 template <class Sink>
 __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
                                               const float v[2], const float C[4], const float D[4], float Jp,
@@ -239,7 +265,7 @@ __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, const Consts
     P2GStencil st;
     const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, st);
     if (e) return e;
-    st.nodes(cf, [&](int i, int j, float dm, float dpx, float dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
+    st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
     return 0;
 }
 
