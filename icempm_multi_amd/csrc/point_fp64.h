@@ -90,10 +90,11 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     };
     // Two separate loops: if the LDS and the global reads fed one loop, the
     // merged value would make every gather wait for all outstanding global
-    // loads, including the prefetch of the next point.  The columns are left
-    // rolled: fully unrolled they keep 18 node values live.
+    // loads, including the prefetch of the next point.  The LDS loop is
+    // unrolled (its 9 reads in flight together, one wait), the global one is
+    // not (18 node values live at once would not fit).
     if (inwin) {
-#pragma unroll 1
+#pragma unroll
         for (int i = 0; i < 3; i++) {
             double ax = 0, ay = 0, bx = 0, by_ = 0;
 #pragma unroll
