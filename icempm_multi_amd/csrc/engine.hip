@@ -205,6 +205,35 @@ int Engine::build_scene_dev(int i, SceneDev &d) const
     c.ind_vel = P.IndVelocity;
     c.ind_x0 = P.indenter_x_initial;
     c.ind_y = P.indenter_y;
+    ConstsF &f = c.f;
+    f.h = (float)c.h;
+    f.hinv = (float)c.hinv;
+    f.dt = (float)c.dt;
+    f.dt_vol_Dpinv = (float)c.dt_vol_Dpinv;
+    f.pmass = (float)c.pmass;
+    f.wmass = (float)c.wmass;
+    f.mu = (float)c.mu;
+    f.kappa = (float)c.kappa;
+    f.ics = (float)c.ics;
+    f.its = (float)c.its;
+    f.iss = (float)c.iss;
+    f.its2 = (float)c.its2;
+    f.beta = (float)c.beta;
+    f.grain_var = (float)c.grain_var;
+    f.s4 = (float)(4. * c.hinv);
+    f.dq_scale = (float)(c.dt * c.hinv * (double)(1LL << c.frac_bits));
+    f.fb = c.frac_bits;
+    f.origin = c.origin;
+    f.origin_y = c.origin_y;
+    {
+        const double pmax = c.ics, pmin = -c.its, dp = pmax - pmin;
+        f.gv033 = (float)(c.grain_var * 0.033);
+        f.gv01 = (float)(c.grain_var * 0.1);
+        f.q0c = (float)(2. * std::sqrt(-pmax * pmin) / dp);
+        f.msq_c = (float)(4. * (1. + 2. * c.beta) / (dp * dp));
+        f.b2 = (float)(1. + 2. * c.beta);
+        f.bpmax = (float)(c.beta * pmax);
+    }
     d.pt_off = s.pt_off;
     d.n_pts = s.n_pts;
     d.cap_pts = s.cap_pts;

@@ -187,10 +187,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
             if (!ok) continue;
             if (ud & kDisabled) continue;
             double v[2], B[4];
+            StressTerms terms;   // stress terms of F, passed from G2P to P2G
+            terms.valid = false;
             if (DO_G2P) {
                 double pp, qq;
                 unsigned e;
-                if (!g2p_point(c, gr, gp, vw, wc0, wr0, win_w, win_h, x, y, v, B, F, Jp, ud, pp, qq, e)) {
+                if (!g2p_point(c, gr, gp, vw, wc0, wr0, win_w, win_h, x, y, v, B, F, Jp, ud, pp, qq, e, terms)) {
                     err = max(err, e);
                     continue;
                 }
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
             }
             if (DO_P2G) {
                 P2GStencil st;
-                const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, st);
+                const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, terms, st);
                 if (e) {
                     err = max(err, e);
                     continue;

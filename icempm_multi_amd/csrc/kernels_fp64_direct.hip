@@ -55,8 +55,9 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     double x, y, F[4], Jp, v[2], B[4], pp, qq;
     load_state(R, pt, x, y, F, Jp);
     unsigned e;
+    StressTerms terms;
     if (!g2p_point(c, (const glb_d *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, 0, 0, 0, 0, x, y, v, B, F, Jp, ud,
-                   pp, qq, e)) {
+                   pp, qq, e, terms)) {
         atomicMax(&a.err[scene], e);
         return;
     }

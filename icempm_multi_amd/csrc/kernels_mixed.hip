@@ -166,11 +166,13 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             }
             if (!ok) continue;
             if (ud & (unsigned)kDisabled) continue;
+            StressTerms terms;   // stress terms of D, passed from G2P to P2G
+            terms.valid = false;
             if (DO_G2P) {
                 float pp, qq;
                 unsigned e;
                 if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, win_w, win_h, xq, yq, v, C, D, Jp, ud, pp, qq,
-                               e, (unsigned)pt, (unsigned)step)) {
+                               e, (unsigned)pt, (unsigned)step, terms)) {
                     err = max(err, e);
                     continue;
                 }
@@ -184,7 +186,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             }
             if (DO_P2G) {
                 P2GStencil st;
-                const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, st);
+                const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, terms, st);
                 if (e) {
                     err = max(err, e);
                     continue;

@@ -54,8 +54,9 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     float D[4], Jp, v[2], C[4], pp, qq;
     load_state(R, pt, xq, yq, D, Jp);
     unsigned e;
+    StressTerms terms;
     if (!g2p_point(c, cf, (const glb_f *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, 0, 0, 0, 0, xq,
-                   yq, v, C, D, Jp, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step)) {
+                   yq, v, C, D, Jp, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step, terms)) {
         atomicMax(&a.err[scene], e);
         return;
     }

@@ -29,6 +29,20 @@ constexpr int kSegShift = 6, kSegNodes = 1 << kSegShift;
 // grid with one flag load per 64 segments.
 inline int update_run_length(long long grid_nodes) { return grid_nodes > (8LL << 20) ? 64 : 16; }
 
+// FP32 constants of the mixed form, converted by the host (the kernels keep
+// them in scalar registers).
+struct ConstsF {
+    float h, hinv, dt, dt_vol_Dpinv, pmass, wmass, mu, kappa;
+    float ics, its, iss, its2, beta, grain_var, s4;   // s4 = 4/h
+    float dq_scale;                                   // dt * hinv * 2^frac_bits
+    int fb, origin, origin_y;
+    // grain-independent terms of the failure surface (kernels.cuh:560-619)
+    float gv033, gv01;   // GrainVariability * 0.033 and * 0.1
+    float q0c;           // 2 sqrt(-pmax pmin) / (pmax - pmin)
+    float msq_c;         // 4 (1 + 2 beta) / (pmax - pmin)^2
+    float b2, bpmax;     // 1 + 2 beta, beta * pmax
+};
+
 // Everything a kernel needs to know about one scene's physics and geometry.
 struct SceneConsts {
     int gridX, gridY, halo, ncols;    // node column col = gx - xoff + halo, gx in [xoff-halo, xoff+gridX+halo)
@@ -42,6 +56,7 @@ struct SceneConsts {
     double ics, its, iss, its2;       // Ice{Compressive,Tensile,Shear,Tensile2}Strength
     double beta, grain_var, tan_phi, dp_threshold_p;
     double ind_rsq, ind_vel, ind_x0, ind_y;
+    ConstsF f;                        // mixed form
 };
 
 struct SceneDev {

@@ -46,6 +46,26 @@ __device__ __forceinline__ void weights(double fx, double w[3])
     w[2] = 0.5 * v2 * v2;
 }
 
+// Terms that ComputePQ (G2P) and the Kirchhoff stress (P2G) share for an
+// intact point, whose F the constitutive step leaves unchanged: 1/Je,
+// dev(F F^T) and p = -kappa/2 (Je^2-1).
+struct StressTerms {
+    double inv_je, d[4], p;
+    bool valid;
+};
+
+// Kirchhoff stress of Wolper 2019 (kernels.cuh:24-34) from the shared terms:
+// the same operations on the same values as the form below.
+__device__ __forceinline__ void kirchhoff(const SceneConsts &c, const StressTerms &st, double tau[4])
+{
+    const double a = c.mu * st.inv_je;
+    const double k = -st.p;
+    tau[0] = a * st.d[0] + k;
+    tau[1] = a * st.d[1];
+    tau[2] = a * st.d[2];
+    tau[3] = a * st.d[3] + k;
+}
+
 // Kirchhoff stress of Wolper 2019 (kernels.cuh:24-34)
 __device__ __forceinline__ void kirchhoff(const SceneConsts &c, const double F[4], double tau[4])
 {
@@ -191,16 +211,26 @@ __device__ __forceinline__ void usvt(const double U[4], double s0, double s1, co
     mul2(UD, Vt, F);
 }
 
-// ComputePQ (kernels.cuh:547-557)
+// ComputePQ (kernels.cuh:547-557); also leaves the terms the stress shares.
 __device__ __forceinline__ void compute_pq(const SceneConsts &c, const double F[4], double &Je, double &p,
-                                           double &q)
+                                           double &q, StressTerms &st)
 {
     Je = det2(F);
     p = -(c.kappa / 2.) * (Je * Je - 1.);
-    double d[4];
-    dev_ffT(F, d);
+    dev_ffT(F, st.d);
+    const double *d = st.d;
     const double nrm = sqrt(d[0] * d[0] + d[2] * d[2] + d[1] * d[1] + d[3] * d[3]);
-    q = kCoeff1 * c.mu * (1. / Je) * nrm;
+    st.inv_je = 1. / Je;
+    q = kCoeff1 * c.mu * st.inv_je * nrm;
+    st.p = p;
+    st.valid = true;
+}
+
+__device__ __forceinline__ void compute_pq(const SceneConsts &c, const double F[4], double &Je, double &p,
+                                           double &q)
+{
+    StressTerms st;
+    compute_pq(c, F, Je, p, q, st);
 }
 
 // CheckIfPointIsInsideFailureSurface + GetParametersForGrain (kernels.cuh:560-619)
@@ -266,20 +296,26 @@ __device__ __forceinline__ void drucker_prager(const SceneConsts &c, double F[4]
 
 // The constitutive branch of partition_kernel_g2p for one point (kernels.cuh:378-391).
 // A = I + dt*C has been applied to F already.
+// The terms the stress shares are left in st (st.valid false when the
+// projection changed F or the point is liquid).
 __device__ __forceinline__ void constitutive(const SceneConsts &c, double F[4], double &Jp_inv, long long &ud,
-                                             const double A[4], double &p_out, double &q_out)
+                                             const double A[4], double &p_out, double &q_out, StressTerms &st)
 {
+    st.valid = false;
     if (ud & kLiquid) {
         Jp_inv *= det2(A);
         p_out = 0; q_out = 0;
         return;
     }
     double Je, p, q;
-    compute_pq(c, F, Je, p, q);
+    compute_pq(c, F, Je, p, q, st);
     p_out = p; q_out = q;
     if (!(ud & kCrushed))
         if (failure(c, (short)ud, p, q)) ud |= kCrushed;
-    if (ud & kCrushed) drucker_prager(c, F, Jp_inv, Je, p, q);
+    if (ud & kCrushed) {
+        st.valid = false;
+        drucker_prager(c, F, Jp_inv, Je, p, q);
+    }
 }
 
 }  // namespace fp64

@@ -20,38 +20,8 @@ namespace mixed {
 
 constexpr float kCoeff1f = 1.41421356237309505f;
 
-// Float constants of a scene, converted once per tile from SceneConsts.
-struct ConstsF {
-    float h, hinv, dt, dt_vol_Dpinv, pmass, wmass, mu, kappa;
-    float ics, its, iss, its2, beta, grain_var, s4;   // s4 = 4/h
-    float dq_scale;                                   // dt * hinv * 2^frac_bits
-    int fb, origin, origin_y;
-};
-
-__device__ __forceinline__ ConstsF to_float(const SceneConsts &c)
-{
-    ConstsF f;
-    f.h = (float)c.h;
-    f.hinv = (float)c.hinv;
-    f.dt = (float)c.dt;
-    f.dt_vol_Dpinv = (float)c.dt_vol_Dpinv;
-    f.pmass = (float)c.pmass;
-    f.wmass = (float)c.wmass;
-    f.mu = (float)c.mu;
-    f.kappa = (float)c.kappa;
-    f.ics = (float)c.ics;
-    f.its = (float)c.its;
-    f.iss = (float)c.iss;
-    f.its2 = (float)c.its2;
-    f.beta = (float)c.beta;
-    f.grain_var = (float)c.grain_var;
-    f.s4 = (float)(4. * c.hinv);
-    f.dq_scale = (float)(c.dt * c.hinv * (double)(1LL << c.frac_bits));
-    f.fb = c.frac_bits;
-    f.origin = c.origin;
-    f.origin_y = c.origin_y;
-    return f;
-}
+// Float constants of a scene (set by the host).
+__device__ __forceinline__ const ConstsF &to_float(const SceneConsts &c) { return c.f; }
 
 // Base node and offset from a fixed-point coordinate: base = (int)(x/h - 0.5)
 // (C truncation, as kernels.cuh:90), fx = x/h - base, both exact.
@@ -82,18 +52,38 @@ __device__ __forceinline__ void strain_measures(const float D[4], float &Jm1, fl
     d1 = e01;
 }
 
+// Stress terms of an intact point, shared by ComputePQ in G2P and the
+// Kirchhoff stress in P2G when the constitutive step leaves D unchanged:
+// a = mu/J, dev(F F^T) = [[d0, d1], [d1, -d0]], k = kappa/2 (J^2-1) = -p.
+struct StressTerms {
+    float a, d0, d1, k;
+    bool valid;
+};
+
+__device__ __forceinline__ void stress_terms(const ConstsF &c, const float D[4], StressTerms &st)
+{
+    float Jm1;
+    strain_measures(D, Jm1, st.d0, st.d1);
+    const float J = 1.f + Jm1;
+    st.a = c.mu / J;
+    st.k = 0.5f * c.kappa * Jm1 * (J + 1.f);
+    st.valid = true;
+}
+
 // Kirchhoff stress (kernels.cuh:24-34): mu/J dev(F F^T) + kappa/2 (J^2-1) I
+__device__ __forceinline__ void kirchhoff(const StressTerms &st, float tau[4])
+{
+    tau[0] = st.a * st.d0 + st.k;
+    tau[1] = st.a * st.d1;
+    tau[2] = st.a * st.d1;
+    tau[3] = -st.a * st.d0 + st.k;
+}
+
 __device__ __forceinline__ void kirchhoff(const ConstsF &c, const float D[4], float tau[4])
 {
-    float Jm1, d0, d1;
-    strain_measures(D, Jm1, d0, d1);
-    const float J = 1.f + Jm1;
-    const float a = c.mu / J;
-    const float k = 0.5f * c.kappa * Jm1 * (J + 1.f);
-    tau[0] = a * d0 + k;
-    tau[1] = a * d1;
-    tau[2] = a * d1;
-    tau[3] = -a * d0 + k;
+    StressTerms st;
+    stress_terms(c, D, st);
+    kirchhoff(st, tau);
 }
 
 __device__ __forceinline__ void water(const ConstsF &c, float J, float tau[4])
@@ -102,34 +92,29 @@ __device__ __forceinline__ void water(const ConstsF &c, float J, float tau[4])
     tau[0] = k; tau[1] = 0; tau[2] = 0; tau[3] = k;
 }
 
-// ComputePQ (kernels.cuh:547-557)
-__device__ __forceinline__ void compute_pq(const ConstsF &c, const float D[4], float &p, float &q)
+// ComputePQ (kernels.cuh:547-557) from the stress terms: p = -k,
+// q = sqrt(2) (mu/J) |dev(F F^T)|.
+__device__ __forceinline__ void compute_pq(const StressTerms &st, float &p, float &q)
 {
-    float Jm1, d0, d1;
-    strain_measures(D, Jm1, d0, d1);
-    const float J = 1.f + Jm1;
-    p = -0.5f * c.kappa * Jm1 * (J + 1.f);
-    q = kCoeff1f * c.mu / J * sqrtf(2.f * (d0 * d0 + d1 * d1));
+    p = -st.k;
+    q = kCoeff1f * st.a * sqrtf(2.f * (st.d0 * st.d0 + st.d1 * st.d1));
 }
 
 // CheckIfPointIsInsideFailureSurface (kernels.cuh:560-619) in FP32
 __device__ __forceinline__ bool failure(const ConstsF &c, int grain, float p, float q)
 {
-    const float var2 = 1.0f + c.grain_var * 0.033f * (float)(-15 + (grain + 3) % 30);
-    const float var3 = 1.0f + c.grain_var * 0.1f * (float)(-10 + (grain + 4) % 11);
-    const float pmax = c.ics, pmin = -c.its;
+    const float var2 = 1.0f + c.gv033 * (float)(-15 + (grain + 3) % 30);
+    const float var3 = 1.0f + c.gv01 * (float)(-10 + (grain + 4) % 11);
     const float qmax = c.iss * var3;
     const float pmin2 = -c.its2 * var2;
-    const float beta = c.beta;
-    const float dp = pmax - pmin;
     if (p < 0) {
         if (p < pmin2) return true;
-        const float q0 = 2.f * sqrtf(-pmax * pmin) * qmax / dp;
+        const float q0 = c.q0c * qmax;
         const float q_limit = -q0 / pmin2 * (p - pmin2);
         return q > q_limit;
     }
-    const float mSq = 4.f * qmax * qmax * (1.f + 2.f * beta) / (dp * dp);
-    const float y = (1.f + 2.f * beta) * q * q + mSq * (p + beta * pmax) * (p - pmax);
+    const float mSq = c.msq_c * (qmax * qmax);
+    const float y = c.b2 * q * q + mSq * (p + c.bpmax) * (p - c.ics);
     return y > 0;
 }
 

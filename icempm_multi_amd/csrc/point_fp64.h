@@ -59,7 +59,8 @@ __device__ __forceinline__ int xcd_block()
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, long long gpitch,
                                           const lds_d *vw, int wc0, int wr0, int win_w,
                                           int win_h, double &x, double &y, double v[2], double B[4], double F[4],
-                                          double &Jp, long long &ud, double &p_out, double &q_out, unsigned &e)
+                                          double &Jp, long long &ud, double &p_out, double &q_out, unsigned &e,
+                                          StressTerms &st)
 {
     int bx, by;
     double fx, fy;
@@ -135,7 +136,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     y += v[1] * c.dt;
     const double A[4] = {1.0 + c.dt * B[0], c.dt * B[1], c.dt * B[2], 1.0 + c.dt * B[3]};
     mul2(A, F, F);
-    constitutive(c, F, Jp, ud, A, p_out, q_out);
+    constitutive(c, F, Jp, ud, A, p_out, q_out, st);
     return true;
 }
 
@@ -170,14 +171,16 @@ struct P2GStencil {
 };
 
 // Stress, base node and weights of a point; returns the error byte (0 = ok).
+// terms: the stress terms of F if the G2P of this launch left them valid.
 __device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, double x, double y, const double v[2],
                                                 const double B[4], const double F[4], double Jp, long long ud,
-                                                P2GStencil &st)
+                                                const StressTerms &terms, P2GStencil &st)
 {
     const bool liquid = (ud & kLiquid) != 0;
     st.mass = liquid ? c.wmass : c.pmass;
     double tau[4];
     if (liquid) water(c, Jp, tau);
+    else if (terms.valid) kirchhoff(c, terms, tau);
     else kirchhoff(c, F, tau);
 #pragma unroll
     for (int k = 0; k < 4; k++) st.S[k] = st.mass * B[k] - c.dt_vol_Dpinv * tau[k];
@@ -202,7 +205,9 @@ __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, double x, do
                                               Sink &&sink)
 {
     P2GStencil st;
-    const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, st);
+    StressTerms none;
+    none.valid = false;
+    const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, none, st);
     if (e) return e;
     st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
     return 0;

@@ -89,20 +89,26 @@ __device__ __forceinline__ unsigned stencil_error(const SceneConsts &c, int bxc,
 // The constitutive branch of partition_kernel_g2p (kernels.cuh:378-391):
 // intact points in FP32; crushed points go through the 2x2 SVD and the
 // Drucker-Prager projection in FP64.
+// The stress terms of the updated D are left in st (st.valid false when the
+// projection changed D or the point is liquid).
 __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF &cf, float D[4], float &Jp,
-                                             unsigned &ud, const float A[4], float &p_out, float &q_out)
+                                             unsigned &ud, const float A[4], float &p_out, float &q_out,
+                                             StressTerms &st)
 {
+    st.valid = false;
     if (ud & (unsigned)kLiquid) {
         Jp *= 1.f + (A[0] + A[3]) + (A[0] * A[3] - A[2] * A[1]);
         p_out = 0; q_out = 0;
         return;
     }
     float p, q;
-    compute_pq(cf, D, p, q);
+    stress_terms(cf, D, st);
+    compute_pq(st, p, q);
     p_out = p; q_out = q;
     if (!(ud & (unsigned)kCrushed))
         if (failure(cf, (short)(ud & 0xffff), p, q)) ud |= (unsigned)kCrushed;
     if (ud & (unsigned)kCrushed) {
+        st.valid = false;
         double F[4] = {1.0 + (double)D[0], (double)D[1], (double)D[2], 1.0 + (double)D[3]};
         double Jpd = Jp, Je, pd, qd;
         fp64::compute_pq(c, F, Je, pd, qd);
@@ -123,7 +129,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
                                           const lds_f *vw, int wc0, int wr0, int win_w,
                                           int win_h, unsigned &xq, unsigned &yq, float v[2], float C[4], float D[4],
                                           float &Jp, unsigned &ud, float &p_out, float &q_out, unsigned &e,
-                                          unsigned point, unsigned step)
+                                          unsigned point, unsigned step, StressTerms &st)
 {
     int bx, by;
     float fx, fy;
@@ -200,7 +206,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     const float n2 = D[2] + A[2] + (A[2] * D[0] + A[3] * D[2]);
     const float n3 = D[3] + A[3] + (A[2] * D[1] + A[3] * D[3]);
     D[0] = n0; D[1] = n1; D[2] = n2; D[3] = n3;
-    constitutive(c, cf, D, Jp, ud, A, p_out, q_out);
+    constitutive(c, cf, D, Jp, ud, A, p_out, q_out, st);
     return true;
 }
 
@@ -232,14 +238,16 @@ struct P2GStencil {
 };
 
 // Stress, base node and weights of a point; returns the error byte (0 = ok).
+// terms: the stress terms of D if the G2P of this launch left them valid.
 __device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
                                                 const float v[2], const float C[4], const float D[4], float Jp,
-                                                unsigned ud, P2GStencil &st)
+                                                unsigned ud, const StressTerms &terms, P2GStencil &st)
 {
     const bool liquid = (ud & (unsigned)kLiquid) != 0;
     st.mass = liquid ? cf.wmass : cf.pmass;
     float tau[4];
     if (liquid) water(cf, Jp, tau);
+    else if (terms.valid) kirchhoff(terms, tau);
     else kirchhoff(cf, D, tau);
 #pragma unroll
     for (int k = 0; k < 4; k++) st.S[k] = st.mass * C[k] - cf.dt_vol_Dpinv * tau[k];
@@ -263,7 +271,9 @@ __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, const Consts
                                               unsigned ud, Sink &&sink)
 {
     P2GStencil st;
-    const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, st);
+    StressTerms none;
+    none.valid = false;
+    const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, none, st);
     if (e) return e;
     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
     return 0;
