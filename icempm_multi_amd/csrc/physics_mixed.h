@@ -25,11 +25,13 @@ __device__ __forceinline__ const ConstsF &to_float(const SceneConsts &c) { retur
 
 // Base node and offset from a fixed-point coordinate: base = (int)(x/h - 0.5)
 // (C truncation, as kernels.cuh:90), fx = x/h - base, both exact.
+// In 32-bit arithmetic: q - off is formed on whichever side is non-negative,
+// and frac = x/h - base (in [-0.5, 1.5) * 2^fb) is exact modulo 2^32.
 __device__ __forceinline__ void base_fx_fixed(unsigned q, int fb, int origin, int &base, float &fx)
 {
-    const long long v = (long long)q - ((long long)origin << fb) - (1LL << (fb - 1));
-    base = v >= 0 ? (int)(v >> fb) : -(int)((-v) >> fb);
-    const long long frac = (long long)q - ((long long)(base + origin) << fb);
+    const unsigned off = ((unsigned)origin << fb) + (1u << (fb - 1));   // (origin + 1/2) * 2^fb
+    base = q >= off ? (int)((q - off) >> fb) : -(int)((off - q) >> fb);
+    const int frac = (int)(q - ((unsigned)(base + origin) << fb));
     fx = (float)frac * __int_as_float((127 - fb) << 23);   // * 2^-fb
 }
 

@@ -62,10 +62,11 @@ struct PtRef {
     __device__ __forceinline__ void setf(int line, long long i, float v) const { u(line, i) = __float_as_uint(v); }
 };
 
-// Uniform [0, 1) from a point index and a substep (murmur3 finalizer).  Used to
-// round the fixed-point position increment stochastically: rounding to nearest
-// biases slowly varying increments of a few units every substep the same way.
-__device__ __forceinline__ float unit_hash(unsigned a, unsigned b)
+// Two uniforms in [0, 1) (16 bits each) from a point index and a substep
+// (murmur3 finalizer).  Used to round the fixed-point position increments
+// stochastically: rounding to nearest biases slowly varying increments of a
+// few units every substep the same way.
+__device__ __forceinline__ void unit_hash2(unsigned a, unsigned b, float &u0, float &u1)
 {
     unsigned h = a * 0x9E3779B1u ^ (b * 0x85EBCA77u + 0x7F4A7C15u);
     h ^= h >> 16;
@@ -73,7 +74,8 @@ __device__ __forceinline__ float unit_hash(unsigned a, unsigned b)
     h ^= h >> 13;
     h *= 0xC2B2AE35u;
     h ^= h >> 16;
-    return (float)(h >> 8) * (1.0f / 16777216.0f);
+    u0 = (float)(h & 0xffffu) * (1.0f / 65536.0f);
+    u1 = (float)(h >> 16) * (1.0f / 65536.0f);
 }
 
 __device__ __forceinline__ unsigned stencil_error(const SceneConsts &c, int bxc, int by)
@@ -197,8 +199,10 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     C[2] = cf.s4 * c10;
     C[3] = cf.s4 * c11;
     // x += v dt in fixed point, rounded stochastically (unbiased)
-    xq += (unsigned)__float2int_rd(v[0] * cf.dq_scale + unit_hash(point, 2 * step));
-    yq += (unsigned)__float2int_rd(v[1] * cf.dq_scale + unit_hash(point, 2 * step + 1));
+    float ux, uy;
+    unit_hash2(point, step, ux, uy);
+    xq += (unsigned)__float2int_rd(v[0] * cf.dq_scale + ux);
+    yq += (unsigned)__float2int_rd(v[1] * cf.dq_scale + uy);
     const float A[4] = {cf.dt * C[0], cf.dt * C[1], cf.dt * C[2], cf.dt * C[3]};
     // D <- (I + A)(I + D) - I = D + A + A D
     const float n0 = D[0] + A[0] + (A[0] * D[0] + A[1] * D[2]);
