@@ -165,25 +165,31 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
             pt = run_start[cidx] + (k - run_pre[cidx]);
             return true;
         };
-        // the state of the next point is loaded while the current one is computed
-        long long pt_n = 0, ud_n = 0;
+        // The state of the next point is loaded while the current one is
+        // computed.  Its index is located one iteration earlier still: the
+        // lookup reads LDS, and LDS reads wait for this wave's earlier LDS
+        // atomics, so it is done after the loads are issued.
+        long long pt_n = 0, ud_n = 0, pt_nn = 0;
         double x_n = 0, y_n = 0, F_n[4] = {0, 0, 0, 0}, Jp_n = 0;
         bool ok_n = n_it > 0 && locate(0, pt_n);
         if (ok_n) {
             ud_n = as_ll(R.at(kLineUtility, pt_n));
             load_state(R, pt_n, x_n, y_n, F_n, Jp_n);
         }
+        bool ok_nn = n_it > 1 && locate(1, pt_nn);
 #pragma unroll 1
         for (int it = 0; it < n_it; it++) {
             const bool ok = ok_n;
             const long long pt = pt_n;
             long long ud = ud_n;
             double x = x_n, y = y_n, F[4] = {F_n[0], F_n[1], F_n[2], F_n[3]}, Jp = Jp_n;
-            ok_n = it + 1 < n_it && locate(it + 1, pt_n);
+            ok_n = ok_nn;
+            pt_n = pt_nn;
             if (ok_n) {
                 ud_n = as_ll(R.at(kLineUtility, pt_n));
                 load_state(R, pt_n, x_n, y_n, F_n, Jp_n);
             }
+            ok_nn = it + 2 < n_it && locate(it + 2, pt_nn);
             if (!ok) continue;
             if (ud & kDisabled) continue;
             double v[2], B[4];

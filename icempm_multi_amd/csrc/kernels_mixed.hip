@@ -144,8 +144,10 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             pt = run_start[cidx] + (k - run_pre[cidx]);
             return true;
         };
-        // the state of the next point is loaded while the current one is computed
-        unsigned pt_n = 0;   // SoA index (points per launch < 2^29)
+        // The state of the next point is loaded while the current one is
+        // computed; its index is located one iteration earlier still (as in
+        // the FP64 form: the LDS lookup waits for earlier LDS atomics).
+        unsigned pt_n = 0, pt_nn = 0;   // SoA indices (points per launch < 2^29)
         unsigned ud_n = 0, xq_n = 0, yq_n = 0;
         float D_n[4] = {0, 0, 0, 0}, Jp_n = 0;
         bool ok_n = n_it > 0 && locate(0, pt_n);
@@ -153,17 +155,20 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             ud_n = R.u(kLineUtility, pt_n);
             load_state(R, pt_n, xq_n, yq_n, D_n, Jp_n);
         }
+        bool ok_nn = n_it > 1 && locate(1, pt_nn);
 #pragma unroll 1
         for (int it = 0; it < n_it; it++) {
             const bool ok = ok_n;
             const unsigned pt = pt_n;
             unsigned ud = ud_n, xq = xq_n, yq = yq_n;
             float D[4] = {D_n[0], D_n[1], D_n[2], D_n[3]}, Jp = Jp_n, v[2], C[4];
-            ok_n = it + 1 < n_it && locate(it + 1, pt_n);
+            ok_n = ok_nn;
+            pt_n = pt_nn;
             if (ok_n) {
                 ud_n = R.u(kLineUtility, pt_n);
                 load_state(R, pt_n, xq_n, yq_n, D_n, Jp_n);
             }
+            ok_nn = it + 2 < n_it && locate(it + 2, pt_nn);
             if (!ok) continue;
             if (ud & (unsigned)kDisabled) continue;
             StressTerms terms;   // stress terms of D, passed from G2P to P2G
