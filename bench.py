@@ -224,7 +224,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the sheet length (tests only)")
     ap.add_argument("--cpu-substeps", type=int, default=300, help="substeps of the CPU baseline sample (~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="replay frames as hipGraphs (no per-kernel events)")
+    ap.add_argument("--mode", choices=["graph", "eager"], default="graph",
+                    help="graph: frames replayed as hipGraphs with hipEvents recorded inside the graph around "
+                         "the launches of one substep in 32; eager: direct launches with events around each")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
     ap.add_argument("--accuracy-steps", type=int, default=1000, help="substeps of the mixed vs FP64 comparison")
     ap.add_argument("--calib-copy", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
@@ -268,7 +270,7 @@ def main():
             chosen, tune_ms = part.autotune(3)
             log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:15]]}")
         frame = P.UpdateEveryNthStep
-        flags_timed = im.STEP_TIMINGS if not args.graph else im.STEP_GRAPH
+        flags_timed = (im.STEP_GRAPH | im.STEP_TIMINGS | im.STEP_SAMPLED) if args.mode == "graph" else im.STEP_TIMINGS
 
         def run(k, flags):
             done = 0
@@ -322,7 +324,7 @@ def main():
         "config": {"workload": f"ice sheet {nx * 0.05:g} x 12.5 m, grid {P.GridXTotal}x{P.GridY}, h=0.05 m, "
                                f"{n} points (4/cell jittered), cylindrical indenter D=1 m at 2 m/s, dt=1e-5 s",
                    "points": n, "grid": [P.GridXTotal, P.GridY], "precision": args.precision,
-                   "tile": primary["tile"], "mode": "graph" if args.graph else "eager+events"},
+                   "tile": primary["tile"], "mode": "graph, events in 1 of 32 substeps" if args.mode == "graph" else "eager, events per launch"},
         "hbm_model_GBps": round(primary["hbm_model_GBps"], 1),
         "hbm_model_frac": round(primary["hbm_model_frac"], 4),
         "roofline": primary["roofline"],

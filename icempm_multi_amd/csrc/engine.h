@@ -98,12 +98,14 @@ private:
     int build_tiles();
     int alloc_bins();
     StepArgs args() const;
-    int launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed);
+    // timed: record hipEvents around the launches of every sample-th substep
+    int launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed, int sample = 1);
     int zero_grid(int scene, int buf);
     int record(int cls, bool timed, hipError_t (*)(void *), void *);
     void timed_begin(bool timed);
     void timed_end(bool timed, int cls);
     int collect_timings();
+    void record(hipEvent_t e);
     void invalidate_graphs();
     int frac_bits_for(const icempm_simparams &P, int &origin) const;
 
@@ -153,9 +155,19 @@ private:
     long long cls_cnt_[kNumCls] = {0};
     float last_step_ms_ = 0;
     hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+    bool capturing_ = false;                 // a graph capture is in progress
+    static constexpr int kTimingSample = 32;  // ICEMPM_STEP_SAMPLED: one substep in 32 is timed
 
-    // graphs keyed by (nsub, start_buf, fused)
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs_;
+    // Graphs keyed by (nsub, start_buf, mode bits).  A timed graph owns the
+    // hipEvents its capture recorded (event record nodes); they are read after
+    // each replay.
+    struct GraphExec {
+        hipGraphExec_t exec = nullptr;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+        std::vector<int> cls;
+    };
+    std::map<std::tuple<int, int, int>, GraphExec> graphs_;
+    const GraphExec *graph_read_ = nullptr;   // the graph launched by the current step()
 };
 
 }  // namespace icempm

@@ -277,7 +277,13 @@ int Engine::push_clocks()
 
 void Engine::invalidate_graphs()
 {
-    for (auto &g : graphs_) hipGraphExecDestroy(g.second);
+    for (auto &g : graphs_) {
+        hipGraphExecDestroy(g.second.exec);
+        for (auto &e : g.second.ev) {
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+    }
     graphs_.clear();
 }
 
@@ -606,10 +612,20 @@ int Engine::download(int scene, double *hssoa, unsigned capacity, int offset, in
 // ---------------------------------------------------------------------------
 // timing helpers
 // ---------------------------------------------------------------------------
+// During a graph capture an event record becomes a node of the graph only
+// with the external flag (a plain record would only order the capture).
+void Engine::record(hipEvent_t e)
+{
+    if (capturing_) hipEventRecordWithFlags(e, stream_, hipEventRecordExternal);
+    else hipEventRecord(e, stream_);
+}
+
 void Engine::timed_begin(bool timed)
 {
     if (!timed) return;
-    if (ev_used_ >= 4096) collect_timings();   // bounds the event pool of long single-phase loops
+    // bounds the event pool of long single-phase loops (no synchronization
+    // while a graph is being captured)
+    if (ev_used_ >= 4096 && !capturing_) collect_timings();
     if (ev_used_ == ev_pool_.size()) {
         hipEvent_t a, b;
         hipEventCreate(&a);
@@ -617,13 +633,13 @@ void Engine::timed_begin(bool timed)
         ev_pool_.push_back({a, b});
         ev_cls_.push_back(0);
     }
-    hipEventRecord(ev_pool_[ev_used_].first, stream_);
+    record(ev_pool_[ev_used_].first);
 }
 
 void Engine::timed_end(bool timed, int cls)
 {
     if (!timed) return;
-    hipEventRecord(ev_pool_[ev_used_].second, stream_);
+    record(ev_pool_[ev_used_].second);
     ev_cls_[ev_used_] = cls;
     ev_used_++;
 }
@@ -884,7 +900,7 @@ int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *ce
 // ---------------------------------------------------------------------------
 // frame stepping
 // ---------------------------------------------------------------------------
-int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed)
+int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed_all, int sample)
 {
     // clocks: the update of substep k reads slot cs and writes the advanced
     // clock to slot cs^1, which the later launches of the substep read
@@ -895,6 +911,7 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
     const int nb_upd = update_blocks(n_chunks_);
     int b = start_buf;
     for (int k = 0; k < nsub; k++) {
+        const bool timed = timed_all && (sample <= 1 || k % sample == 0);
         StepArgs au = a;
         au.clocks = slot[cs];
         au.clocks_next = slot[cs ^ 1];
@@ -956,7 +973,8 @@ int Engine::step(int nsub, int flags)
     const bool direct = (flags & ICEMPM_STEP_DIRECT_P2G) || !bins_valid_;
     const bool fused = !(flags & ICEMPM_STEP_UNFUSED);
     const bool timed = (flags & ICEMPM_STEP_TIMINGS) != 0;
-    const bool use_graph = (flags & ICEMPM_STEP_GRAPH) && !timed;
+    const bool use_graph = (flags & ICEMPM_STEP_GRAPH) != 0;
+    const int sample = (timed && (flags & ICEMPM_STEP_SAMPLED)) ? kTimingSample : 1;
 
     int start_buf = 0;
     if (!direct) {
@@ -972,31 +990,61 @@ int Engine::step(int nsub, int flags)
     HIPCHK(hipEventRecord(ev_a_, stream_));
     const int slot0 = clock_slot_;
     if (use_graph) {
-        const auto key = std::make_tuple(nsub, start_buf, (direct ? 2 : 0) | (fused ? 1 : 0) | (slot0 << 2));
+        const int tmode = timed ? (sample > 1 ? 2 : 1) : 0;
+        const auto key = std::make_tuple(nsub, start_buf,
+                                         (direct ? 2 : 0) | (fused ? 1 : 0) | (slot0 << 2) | (tmode << 3));
         auto it = graphs_.find(key);
         if (it == graphs_.end()) {
+            // The capture records the timing events into the graph: it takes
+            // them from the pool (emptied first) and keeps them.
+            if (timed) RC(collect_timings());
             HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_substeps(nsub, start_buf, fused, direct, false);
+            capturing_ = true;
+            const int rc = launch_substeps(nsub, start_buf, fused, direct, timed, sample);
+            capturing_ = false;
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(stream_, &g);
-            if (rc != ICEMPM_OK) { if (g) hipGraphDestroy(g); return rc; }
-            if (ec != hipSuccess) return hipfail(ec, "hipStreamEndCapture");
-            hipGraphExec_t ge = nullptr;
-            const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            GraphExec gx;
+            gx.ev.assign(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
+            gx.cls.assign(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
+            ev_pool_.erase(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
+            ev_cls_.erase(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
+            ev_used_ = 0;
+            auto drop = [&]() {
+                for (auto &e : gx.ev) {
+                    hipEventDestroy(e.first);
+                    hipEventDestroy(e.second);
+                }
+            };
+            if (rc != ICEMPM_OK) { if (g) hipGraphDestroy(g); drop(); return rc; }
+            if (ec != hipSuccess) { drop(); return hipfail(ec, "hipStreamEndCapture"); }
+            const hipError_t ei = hipGraphInstantiate(&gx.exec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
-            if (ei != hipSuccess) return hipfail(ei, "hipGraphInstantiate");
-            it = graphs_.emplace(key, ge).first;
+            if (ei != hipSuccess) { drop(); return hipfail(ei, "hipGraphInstantiate"); }
+            it = graphs_.emplace(key, gx).first;
         }
-        HIPCHK(hipGraphLaunch(it->second, stream_));
+        HIPCHK(hipGraphLaunch(it->second.exec, stream_));
+        graph_read_ = &it->second;
     } else {
-        RC(launch_substeps(nsub, start_buf, fused, direct, timed));
+        RC(launch_substeps(nsub, start_buf, fused, direct, timed, sample));
     }
     clock_slot_ = slot0 ^ (nsub & 1);   // a replayed graph advances the slot like a direct launch
     HIPCHK(hipEventRecord(ev_b_, stream_));
     HIPCHK(hipEventSynchronize(ev_b_));
     HIPCHK(hipEventElapsedTime(&last_step_ms_, ev_a_, ev_b_));
     last_step_ms_ /= nsub;
-    if (timed) RC(collect_timings());
+    if (timed && use_graph) {
+        // the replay has completed (ev_b_ above): read its events
+        const GraphExec &gx = *graph_read_;
+        for (size_t i = 0; i < gx.ev.size(); i++) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, gx.ev[i].first, gx.ev[i].second));
+            cls_ms_[gx.cls[i]] += ms;
+            cls_cnt_[gx.cls[i]]++;
+        }
+    } else if (timed) {
+        RC(collect_timings());
+    }
 
     for (auto &s : scenes_) {
         if (direct) {

@@ -88,3 +88,4 @@ STEP_RESORT = 0x2
 STEP_TIMINGS = 0x4
 STEP_DIRECT_P2G = 0x8
 STEP_UNFUSED = 0x10
+STEP_SAMPLED = 0x20

@@ -123,6 +123,10 @@ typedef struct icempm_simparams {
 #define ICEMPM_STEP_TIMINGS     0x4   /* record per-phase hipEvents (timing_* fields) */
 #define ICEMPM_STEP_DIRECT_P2G  0x8   /* force the unbinned one-thread-per-point P2G */
 #define ICEMPM_STEP_UNFUSED     0x10  /* keep G2P and the next P2G as separate launches */
+#define ICEMPM_STEP_SAMPLED     0x20  /* with TIMINGS: record hipEvents around the launches of one
+                                         substep in 32 only (substeps 0, 32, ... of each call; kernel
+                                         stats then count those launches); with GRAPH|TIMINGS the
+                                         events are recorded inside the graph */
 
 /* Library identification and the last error text of the calling thread. */
 const char *icempm_version(void);

@@ -111,7 +111,10 @@ def test_phases_binned_match_oracle():
 
 
 @pytest.mark.parametrize("flags,sort", [(im.STEP_DIRECT_P2G, False), (im.STEP_UNFUSED, True), (0, True),
-                                        (im.STEP_GRAPH, True), (im.STEP_TIMINGS, True)])
+                                        (im.STEP_GRAPH, True), (im.STEP_TIMINGS, True),
+                                        (im.STEP_GRAPH | im.STEP_TIMINGS, True),
+                                        (im.STEP_GRAPH | im.STEP_TIMINGS | im.STEP_SAMPLED, True),
+                                        (im.STEP_TIMINGS | im.STEP_SAMPLED, True)])
 def test_substeps_match_oracle(flags, sort):
     P, h, n = small_block(seed=0x1CE0200)
     with_motion(P, h, n, seed=8)
@@ -127,6 +130,28 @@ def test_substeps_match_oracle(flags, sort):
     assert_close(download(part, h), ref, n, rtol=1e-9, what=f"flags={flags}")
     tg, sg = part.clock
     assert sg == 20 and tg == t
+
+
+@pytest.mark.parametrize("graph", [0, im.STEP_GRAPH])
+def test_sampled_timings_count_launches(graph):
+    """Events around the launches of substeps 0, 32, 64 of a 70-substep call:
+    the P2G of substep 0, three updates and three fused launches; a replay of
+    the timed graph adds as many again."""
+    P, h, n = small_block(seed=0x1CE0201)
+    part = make_partition(P, h, n)
+    part.sort_points()
+    part.reset_indenter_force_accumulator()
+    flags = graph | im.STEP_TIMINGS | im.STEP_SAMPLED
+    part.step(70, flags)
+    st = part.kernel_stats()
+    assert st["p2g"][1] == 1 and st["update"][1] == 3 and st["fused_g2p_p2g"][1] == 3 and st["g2p"][1] == 0
+    assert all(st[k][0] > 0 for k in ("p2g", "update", "fused_g2p_p2g"))
+    part.step(70, flags)
+    st = part.kernel_stats()
+    assert st["p2g"][1] == 2 and st["update"][1] == 6 and st["fused_g2p_p2g"][1] == 6
+    part.step(70, im.STEP_GRAPH)   # untimed: no more events
+    assert part.kernel_stats()["fused_g2p_p2g"][1] == 6
+    assert part.error_code == 0
 
 
 @pytest.mark.parametrize("dynamic", [0, 1])
