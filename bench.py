@@ -322,7 +322,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"ice sheet {nx * 0.05:g} x 12.5 m, grid {P.GridXTotal}x{P.GridY}, h=0.05 m, "
-                               f"{n} points (4/cell jittered), cylindrical indenter D=1 m at 2 m/s, dt=1e-5 s",
+                               f"{n} points (Poisson-disk, 4/cell on average), cylindrical indenter D=1 m at 2 m/s, "
+                               f"dt=1e-5 s",
                    "points": n, "grid": [P.GridXTotal, P.GridY], "precision": args.precision,
                    "tile": primary["tile"], "mode": "graph, events in 1 of 32 substeps" if args.mode == "graph" else "eager, events per launch"},
         "hbm_model_GBps": round(primary["hbm_model_GBps"], 1),

@@ -81,6 +81,8 @@ _SIGS = {
     "icempm_batch_last_step_ms": (C.c_int, [_VP, C.POINTER(C.c_float)]),
     "icempm_batch_get_kernel_stats": (C.c_int, [_VP, _D, C.POINTER(C.c_longlong), C.c_int]),
     "icempm_batch_reset_kernel_stats": (C.c_int, [_VP]),
+    "icempm_poisson_disk": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, _D, _D,
+                                      C.c_longlong, C.POINTER(C.c_longlong)]),
     "icempm_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
 }
 
@@ -152,6 +154,21 @@ def prepare_raw_points(P, hssoa, n_points, volume):
     _check(lib().icempm_prepare_raw_points(C.byref(P), _dptr(hssoa), hssoa.shape[1], n_points,
                                            C.c_float(volume), C.byref(kept)))
     return kept.value
+
+
+def poisson_disk(r, x0, y0, x1, y1, seed):
+    """Poisson-disk sample of [x0, x1) x [y0, y1) with minimum distance r
+    (generator/2d/grainprocessor2d.cpp:40-56, Bridson's algorithm); returns x, y."""
+    # at most one point per disk of radius r/2: hexagonal packing bounds the count
+    # (the library rejects r <= 0 and empty boxes)
+    ok = r > 0 and x1 > x0 and y1 > y0
+    cap = int(1.2 * (x1 - x0 + r) * (y1 - y0 + r) / (r * r)) + 16 if ok else 16
+    x = np.empty(cap)
+    y = np.empty(cap)
+    n = C.c_longlong(0)
+    _check(lib().icempm_poisson_disk(r, x0, y0, x1, y1, seed & 0xFFFFFFFFFFFFFFFF, _dptr(x), _dptr(y), cap,
+                                     C.byref(n)))
+    return x[:n.value].copy(), y[:n.value].copy()
 
 
 # ---------------------------------------------------------------- partition

@@ -8,8 +8,9 @@ four inputs from a SplitMix64 stream and hand them to the library's
 
 Workloads (all FP64 inputs; sizes in cells of side h):
   block   40 x 10 m block, grid 256 x 128, h = 0.25, 4 jittered points/cell
-  sheet   200 x 12.5 m sheet, grid 4096 x 1024, h = 0.05, 4 points/cell,
-          cylindrical indenter D = 1 m, depth 0.25 m, moving at 2 m/s
+  sheet   200 x 12.5 m sheet, grid 4096 x 1024, h = 0.05, Poisson-disk cloud of
+          4 points/cell on average (or jittered), cylindrical indenter D = 1 m,
+          depth 0.25 m, moving at 2 m/s
   batch   many blocks of log-uniform point counts and per-scene E, cohesion
   floes   fields of seeded convex polygonal floes, power-law sizes
 """
@@ -63,6 +64,20 @@ def jittered_block(nx_cells, ny_cells, h, seed, per_side=2, jitter=0.25, x0=0.0,
     return x, y
 
 
+# Points per r^2 of area that the Poisson-disk sampler fills (measured on
+# large boxes; the reference's generator uses 0.656 with its sampler).
+POISSON_FILL = 0.62
+
+
+def poisson_block(nx_cells, ny_cells, h, seed, per_cell=4, x0=0.0, y0=0.0):
+    """Poisson-disk cloud of about per_cell points per cell over the block, as the
+    reference's generator makes it (grainprocessor2d.cpp:40-56): minimum distance
+    r = sqrt(POISSON_FILL * area / n), n = per_cell * cells (r = 0.39 h at 4 per cell)."""
+    w, hgt = nx_cells * h, ny_cells * h
+    r = math.sqrt(POISSON_FILL * w * hgt / (per_cell * nx_cells * ny_cells))
+    return capi.poisson_disk(r, x0, y0, x0 + w, y0 + hgt, seed)
+
+
 def voronoi_grains(x, y, n_grains, seed):
     """Grain id of each point: index of the nearest of n_grains seeded sites."""
     from scipy.spatial import cKDTree
@@ -106,8 +121,10 @@ def block_scene(seed=0x1CE0000, E=9e9, cohesion=1e5, phi=30.0, nx=160, ny=40, h=
 
 
 def sheet_scene(seed=0x1CE0001, E=9e9, cohesion=1e5, phi=30.0, nx=4000, ny=250, h=0.05, grid=(4096, 1024),
-                dt=1e-5, halo=8, ind_diameter=1.0, ind_depth=0.25, ind_velocity=2.0, n_grains=1000, **extra):
-    """The 200 x 12.5 m sheet (grid 4096 x 1024, h = 0.05 m, ~4.0M points) with a moving cylindrical indenter."""
+                dt=1e-5, halo=8, ind_diameter=1.0, ind_depth=0.25, ind_velocity=2.0, n_grains=1000,
+                sampler="poisson", **extra):
+    """The 200 x 12.5 m sheet (grid 4096 x 1024, h = 0.05 m, ~4.0M points) with a moving cylindrical
+    indenter.  sampler: "poisson" (Poisson-disk, 4 points per cell on average) or "jittered"."""
     tan_phi = math.tan(phi * math.pi / 180)
     kw = dict(GridX=grid[0], GridY=grid[1], GridXDimension=grid[0] * h, InitialTimeStep=dt, YoungsModulus=E,
               PoissonsRatio=0.3, Density=980.0, DP_phi=phi, DP_threshold_p=cohesion / tan_phi, SetupType=0,
@@ -115,7 +132,12 @@ def sheet_scene(seed=0x1CE0001, E=9e9, cohesion=1e5, phi=30.0, nx=4000, ny=250, 
               nPartitions=1)
     kw.update(extra)
     P = make_params(**kw)
-    x, y = jittered_block(nx, ny, h, seed)
+    if sampler == "poisson":
+        x, y = poisson_block(nx, ny, h, seed)
+    elif sampler == "jittered":
+        x, y = jittered_block(nx, ny, h, seed)
+    else:
+        raise ValueError(f"unknown sampler {sampler!r}")
     g = voronoi_grains(x, y, n_grains, seed)
     hssoa, n = load_raw_points(P, x, y, g, nx * h * ny * h)
     return P, hssoa, n

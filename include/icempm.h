@@ -289,6 +289,18 @@ int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep);
 int icempm_batch_get_kernel_stats(icempm_batch *b, double *ms, long long *count, int n);
 int icempm_batch_reset_kernel_stats(icempm_batch *b);
 
+/* ---- scene generation ---------------------------------------------------
+ * Poisson-disk sample of the box [x0, x1) x [y0, y1) with minimum distance r:
+ * the point cloud of GrainProcessor2D::GenerateBlock/GenerateFloe
+ * (generator/2d/grainprocessor2d.cpp:40-56, 250-262), which call
+ * thinks::PoissonDiskSampling (generator/poisson_disk_sampling/
+ * poisson_disk_sampling.h) with r = sqrt(0.656 * area / n).  Bridson's
+ * algorithm with 30 candidates per active sample, SplitMix64 from `seed`
+ * (deterministic).  Writes the points to x[], y[] (room for cap each) and
+ * their count to *n; ICEMPM_ERR_CAPACITY if more than cap were placed. */
+int icempm_poisson_disk(double r, double x0, double y0, double x1, double y1, uint64_t seed, double *x, double *y,
+                        long long cap, long long *n);
+
 #ifdef __cplusplus
 }
 #endif

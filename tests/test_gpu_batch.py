@@ -24,7 +24,7 @@ def mixed_bag(seed=0x1CE0A00):
     for i, s in enumerate(specs):
         out.append(scenes.block_scene(seed=seed + i, n_grains=30, **s))
     out.append(scenes.sheet_scene(seed=seed + 9, nx=100, ny=16, grid=(160, 48), h=0.05, ind_diameter=0.5,
-                                  ind_depth=0.15, n_grains=30))
+                                  ind_depth=0.15, n_grains=30, sampler="jittered"))
     for P, h, n in out:                      # some motion so every term is exercised
         r = np.random.default_rng(n)
         h[6, :n] = 0.3 * np.sin(h[4, :n])

@@ -249,7 +249,7 @@ def test_error_codes(where, code):
 
 def test_indenter_force_matches_oracle():
     P, h, n = scenes.sheet_scene(nx=120, ny=20, grid=(192, 64), h=0.05, ind_diameter=0.6, ind_depth=0.2,
-                                 n_grains=40)
+                                 n_grains=40, sampler="jittered")
     ref = h.copy()
     part = make_partition(P, h, n)
     part.sort_points()
@@ -304,7 +304,7 @@ def test_autotune_keeps_state():
 
 
 def test_mass_conservation_sheet_fused():
-    P, h, n = scenes.sheet_scene(nx=400, ny=40, grid=(512, 128), h=0.05, n_grains=100)
+    P, h, n = scenes.sheet_scene(nx=400, ny=40, grid=(512, 128), h=0.05, n_grains=100, sampler="jittered")
     part = make_partition(P, h, n)
     part.sort_points()
     part.step(50, im.STEP_GRAPH)

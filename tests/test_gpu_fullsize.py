@@ -47,7 +47,7 @@ def check_bins(part, P, state, n, fixed=None):
 @pytest.mark.parametrize("precision", [im.PRECISION_FP64, im.PRECISION_MIXED])
 def test_sheet_4m_binning_mass_and_state(sheet, precision):
     P, h, n = sheet
-    assert n == 4_000_000
+    assert abs(n - 4_000_000) < 40_000            # Poisson-disk cloud: 4 points per cell on average
     part = im.Partition(0, 0, precision)
     part.allocate(h.shape[1], P.GridXTotal)
     part.update_constants(P)

@@ -181,7 +181,7 @@ def test_mixed_error_code():
 
 
 def test_mixed_indenter_sheet_runs():
-    P, h, n = scenes.sheet_scene(nx=400, ny=40, grid=(512, 128), h=0.05, n_grains=100)
+    P, h, n = scenes.sheet_scene(nx=400, ny=40, grid=(512, 128), h=0.05, n_grains=100, sampler="jittered")
     ref = h.copy()
     part = make(P, h, n, im.PRECISION_MIXED)
     part.sort_points()
