@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
     ap.add_argument("--tile", default=None, help="TXxTY tile shape instead of the autotuner's choice")
     ap.add_argument("--workload", choices=["sheet", "batch", "floes"], default="sheet")
+    ap.add_argument("--sampler", choices=["poisson", "jittered"], default="poisson",
+                    help="point cloud of the sheet: Poisson-disk (the reference generator's) or jittered lattice")
     ap.add_argument("--scenes", type=int, default=None, help="scenes of the batch (64) or floes (8) workload")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -251,7 +253,7 @@ def main():
 
     nx = int(round(4000 * args.scale))
     t0 = time.perf_counter()
-    P, hssoa, n = scenes.sheet_scene(nx=nx)
+    P, hssoa, n = scenes.sheet_scene(nx=nx, sampler=args.sampler)
     log(f"scene: {n} points, grid {P.GridXTotal}x{P.GridY}, h={P.cellsize}, setup {time.perf_counter() - t0:.1f}s")
     nn = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY
 
@@ -322,7 +324,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"ice sheet {nx * 0.05:g} x 12.5 m, grid {P.GridXTotal}x{P.GridY}, h=0.05 m, "
-                               f"{n} points (Poisson-disk, 4/cell on average), cylindrical indenter D=1 m at 2 m/s, "
+                               f"{n} points ({'Poisson-disk' if args.sampler == 'poisson' else 'jittered lattice'}, 4/cell on average), "
+                               f"cylindrical indenter D=1 m at 2 m/s, "
                                f"dt=1e-5 s",
                    "points": n, "grid": [P.GridXTotal, P.GridY], "precision": args.precision,
                    "tile": primary["tile"], "mode": "graph, events in 1 of 32 substeps" if args.mode == "graph" else "eager, events per launch"},
