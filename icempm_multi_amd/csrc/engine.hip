@@ -477,7 +477,10 @@ int Engine::set_tile_shape(int tx, int ty, int margin)
 {
     if (tx < 1 || tx > 32 || ty < 1 || ty > 1024 || margin < 0 || margin > 4)
         return fail(ICEMPM_ERR_ARGUMENT, "tile shape out of range");
-    const size_t lds = (size_t)(tx + 2 + 2 * margin) * (ty + 2 + 2 * margin) * 5 * sizeof(double);
+    // LDS of the fused launch per window node: (vx, vy) and two FP64 (m, px, py)
+    // accumulators, 64 B in FP64 and 56 B in the mixed form (FP32 velocities)
+    const size_t per_node = precision_ == ICEMPM_PRECISION_MIXED ? 56 : 64;
+    const size_t lds = (size_t)(tx + 2 + 2 * margin) * (ty + 2 + 2 * margin) * per_node;
     if (lds > 64 * 1024) return fail(ICEMPM_ERR_ARGUMENT, "tile window exceeds the LDS budget");
     if (tx == tx_ && ty == ty_ && margin == margin_) return ICEMPM_OK;
     tx_ = tx;

@@ -50,7 +50,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
     const int WN = win_w * win_h;
     lds_d *lds = (lds_d *)lds_raw;
     lds_d *vw = lds;                            // 2*WN: (vx, vy) per window node
-    lds_d *acc = lds + (DO_G2P ? 2 * WN : 0);   // 3*WN: (m, px, py) per window node
+    // Two accumulation windows of 3*WN, (m, px, py) per node: even lanes add
+    // into the first, odd lanes into the second.  The adjacent lanes of a
+    // 16-lane row hold points 4 apart in the sorted order, which share a cell
+    // (and so every LDS address) whenever the cell holds more than 4 points;
+    // with a window each they never collide.  The flush adds the two.
+    lds_d *acc = lds + (DO_G2P ? 2 * WN : 0);
+    lds_d *const acc_lane = acc + ((threadIdx.x & 1) ? 3 * WN : 0);
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(double *)a.pts, a.pitch};
@@ -126,7 +132,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                 vw[2 * i] = vx;
                 vw[2 * i + 1] = vy;
             }
-            if (DO_P2G) { acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0; }
+            if (DO_P2G) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc[3 * i + q] = acc[3 * WN + 3 * i + q] = 0;
+            }
         }
         __syncthreads();
         if (tid == 0) {
@@ -219,7 +228,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                 }
                 const int lx = st.bxc - wc0, ly = st.by - wr0;
                 if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
-                    lds_d *const p0 = acc + 3 * (lx * win_h + ly);
+                    lds_d *const p0 = acc_lane + 3 * (lx * win_h + ly);
                     st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
                         lds_d *const p = p0 + 3 * (i * win_h + j);
                         lds_add(p, dm);
@@ -230,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
                     // the stencil leaves the window (a point that drifted since the sort)
                     st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
                         if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
-                            lds_d *const p = acc + 3 * ((lx + i) * win_h + ly + j);
+                            lds_d *const p = acc_lane + 3 * ((lx + i) * win_h + ly + j);
                             lds_add(p, dm);
                             lds_add(p + 1, dpx);
                             lds_add(p + 2, dpy);
@@ -249,14 +258,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))
         if (DO_P2G) {
             __syncthreads();
             for (int i = tid; i < WN; i += kBlock) {
-                const double m = acc[3 * i];
+                const double m = acc[3 * i] + acc[3 * WN + 3 * i];
                 if (m == 0) continue;
                 const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
                 glb_add(gw + gi, m);
-                glb_add(gw + gp + gi, acc[3 * i + 1]);
-                glb_add(gw + 2 * gp + gi, acc[3 * i + 2]);
+                glb_add(gw + gp + gi, acc[3 * i + 1] + acc[3 * WN + 3 * i + 1]);
+                glb_add(gw + 2 * gp + gi, acc[3 * i + 2] + acc[3 * WN + 3 * i + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }
@@ -352,7 +361,7 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
     const size_t WN = (size_t)a.win_w * a.win_h;
     size_t n = 0;
     if (mode & kTileG2P) n += 2 * WN;
-    if (mode & kTileP2G) n += 3 * WN;
+    if (mode & kTileP2G) n += 6 * WN;   // two accumulation windows
     return n * sizeof(double);
 }
 

@@ -30,9 +30,12 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
 
     const int win_h = WHC > 0 ? WHC : a.win_h, win_w = a.win_w;
     const int WN = win_w * win_h;
-    // accumulators (FP64, (m, px, py) per window node) first, then the FP32 velocity window
+    // Two FP64 accumulation windows of 3*WN, (m, px, py) per node, one for
+    // even and one for odd lanes as in the FP64 form, then the FP32 velocity
+    // window.
     lds_d *acc = (lds_d *)lds_raw;
-    lds_f *vw = (lds_f *)(acc + (DO_P2G ? 3 * WN : 0));   // 2*WN: (vx, vy) per window node
+    lds_d *const acc_lane = acc + ((threadIdx.x & 1) ? 3 * WN : 0);
+    lds_f *vw = (lds_f *)(acc + (DO_P2G ? 6 * WN : 0));   // 2*WN: (vx, vy) per window node
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(unsigned *)a.pts, a.pitch};
@@ -107,7 +110,10 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 vw[2 * i] = vx;
                 vw[2 * i + 1] = vy;
             }
-            if (DO_P2G) { acc[3 * i] = 0; acc[3 * i + 1] = 0; acc[3 * i + 2] = 0; }
+            if (DO_P2G) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc[3 * i + q] = acc[3 * WN + 3 * i + q] = 0;
+            }
         }
         __syncthreads();
         if (tid == 0) {
@@ -198,7 +204,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                 }
                 const int lx = st.bxc - wc0, ly = st.by - wr0;
                 if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
-                    lds_d *const p0 = acc + 3 * (lx * win_h + ly);
+                    lds_d *const p0 = acc_lane + 3 * (lx * win_h + ly);
                     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
                         lds_d *const p = p0 + 3 * (i * win_h + j);
                         lds_add(p, dm);
@@ -209,7 +215,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
                     // the stencil leaves the window (a point that drifted since the sort)
                     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
                         if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
-                            lds_d *const p = acc + 3 * ((lx + i) * win_h + ly + j);
+                            lds_d *const p = acc_lane + 3 * ((lx + i) * win_h + ly + j);
                             lds_add(p, dm);
                             lds_add(p + 1, dpx);
                             lds_add(p + 2, dpy);
@@ -228,14 +234,14 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         if (DO_P2G) {
             __syncthreads();
             for (int i = tid; i < WN; i += kBlock) {
-                const double m = acc[3 * i];
+                const double m = acc[3 * i] + acc[3 * WN + 3 * i];
                 if (m == 0) continue;
                 const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
                 twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, m);
-                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, acc[3 * i + 1]);
-                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, acc[3 * i + 2]);
+                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, acc[3 * i + 1] + acc[3 * WN + 3 * i + 1]);
+                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, acc[3 * i + 2] + acc[3 * WN + 3 * i + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }
@@ -327,7 +333,7 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
 {
     const size_t WN = (size_t)a.win_w * a.win_h;
     size_t n = 0;
-    if (mode & kTileP2G) n += 3 * WN * sizeof(double);
+    if (mode & kTileP2G) n += 6 * WN * sizeof(double);   // two accumulation windows
     if (mode & kTileG2P) n += 2 * WN * sizeof(float);
     return n;
 }
