@@ -27,10 +27,10 @@ namespace fp64 {
 
 // ---------------------------------------------------------------------------
 // Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
-// 2 waves per SIMD (up to 256 VGPRs; the fused kernel needs about 204): at 3
-// waves it spilled about 20 VGPRs in the point loop, and every spill reload
-// waited for the prefetched loads of the next point (204 vs 230 us per
-// launch on the 4M-point sheet).
+// 2 waves per SIMD (up to 256 VGPRs; the fused kernel needs about 236): at 3
+// waves (168 VGPRs) it spills about 36 VGPRs in the point loop, and a spill
+// reload waits for every outstanding load, the prefetch of the next point
+// included (measured slower on the 4M-point sheet).
 // ---------------------------------------------------------------------------
 //
 // WHC is the window height (ty + 2 + 2*margin) when fixed at compile time for
