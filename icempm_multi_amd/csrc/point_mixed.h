@@ -160,9 +160,10 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
         c11 += w * by_;
     };
     // separate LDS and global loops, so that the LDS gather does not wait for
-    // the global loads in flight (the prefetch of the next point)
+    // the global loads in flight (the prefetch of the next point); the LDS
+    // loop is unrolled (its 9 reads in flight together)
     if (inwin) {
-#pragma unroll 1
+#pragma unroll
         for (int i = 0; i < 3; i++) {
             float ax = 0, ay = 0, bx_ = 0, by_ = 0;
 #pragma unroll
