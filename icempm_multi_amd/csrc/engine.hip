@@ -542,7 +542,7 @@ int Engine::upload(int scene, const double *hssoa, unsigned capacity, int offset
             };
             putf(kLineP, L(kLineP, i));
             putf(kLineQ, L(kLineQ, i));
-            putf(kLineJp, L(kLineJp, i));
+            putf(kLineJp, L(kLineJp, i) - 1.0);   // the mixed form stores Jp_inv - 1
             putf(kLineVelX, L(kLineVelX, i));
             putf(kLineVelX + 1, L(kLineVelX + 1, i));
             putf(kLineFe00, L(kLineFe00, i) - 1.0);
@@ -598,7 +598,7 @@ int Engine::download(int scene, double *hssoa, unsigned capacity, int offset, in
                 put(kLinePosX + k, ((double)buf[(size_t)(kLinePosX + k) * n + i] * inv - org[k]) * d.c.h);
             put(kLineP, getf(kLineP));
             put(kLineQ, getf(kLineQ));
-            put(kLineJp, getf(kLineJp));
+            put(kLineJp, getf(kLineJp) + 1.0);
             put(kLineVelX, getf(kLineVelX));
             put(kLineVelX + 1, getf(kLineVelX + 1));
             put(kLineFe00, getf(kLineFe00) + 1.0);

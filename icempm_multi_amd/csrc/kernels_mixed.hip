@@ -155,11 +155,11 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         // the FP64 form: the LDS lookup waits for earlier LDS atomics).
         unsigned pt_n = 0, pt_nn = 0;   // SoA indices (points per launch < 2^29)
         unsigned ud_n = 0, xq_n = 0, yq_n = 0;
-        float D_n[4] = {0, 0, 0, 0}, Jp_n = 0;
+        float D_n[4] = {0, 0, 0, 0}, Jm1_n = 0;
         bool ok_n = n_it > 0 && locate(0, pt_n);
         if (ok_n) {
             ud_n = R.u(kLineUtility, pt_n);
-            load_state(R, pt_n, xq_n, yq_n, D_n, Jp_n);
+            load_state(R, pt_n, xq_n, yq_n, D_n, Jm1_n);
         }
         bool ok_nn = n_it > 1 && locate(1, pt_nn);
 #pragma unroll 1
@@ -167,12 +167,12 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             const bool ok = ok_n;
             const unsigned pt = pt_n;
             unsigned ud = ud_n, xq = xq_n, yq = yq_n;
-            float D[4] = {D_n[0], D_n[1], D_n[2], D_n[3]}, Jp = Jp_n, v[2], C[4];
+            float D[4] = {D_n[0], D_n[1], D_n[2], D_n[3]}, Jm1 = Jm1_n, v[2], C[4];
             ok_n = ok_nn;
             pt_n = pt_nn;
             if (ok_n) {
                 ud_n = R.u(kLineUtility, pt_n);
-                load_state(R, pt_n, xq_n, yq_n, D_n, Jp_n);
+                load_state(R, pt_n, xq_n, yq_n, D_n, Jm1_n);
             }
             ok_nn = it + 2 < n_it && locate(it + 2, pt_nn);
             if (!ok) continue;
@@ -182,13 +182,13 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             if (DO_G2P) {
                 float pp, qq;
                 unsigned e;
-                if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, win_w, win_h, xq, yq, v, C, D, Jp, ud, pp, qq,
+                if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, win_w, win_h, xq, yq, v, C, D, Jm1, ud, pp, qq,
                                e, (unsigned)pt, (unsigned)step, terms)) {
                     err = max(err, e);
                     continue;
                 }
-                if (DO_P2G) store_state_fused(R, pt, xq, yq, D, Jp, ud, record, pp, qq);
-                else store_state(R, pt, xq, yq, v, D, C, Jp, ud, record, pp, qq);
+                if (DO_P2G) store_state_fused(R, pt, xq, yq, D, Jm1, ud, record, pp, qq);
+                else store_state(R, pt, xq, yq, v, D, C, Jm1, ud, record, pp, qq);
             } else {
                 v[0] = R.f(kLineVelX, pt);
                 v[1] = R.f(kLineVelX + 1, pt);
@@ -197,7 +197,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             }
             if (DO_P2G) {
                 P2GStencil st;
-                const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, terms, st);
+                const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jm1, ud, terms, st);
                 if (e) {
                     err = max(err, e);
                     continue;

@@ -20,8 +20,8 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
     if (ud & (unsigned)kDisabled) return;
     const ConstsF cf = to_float(c);
     unsigned xq, yq;
-    float D[4], Jp, v[2], C[4];
-    load_state(R, pt, xq, yq, D, Jp);
+    float D[4], Jm1, v[2], C[4];
+    load_state(R, pt, xq, yq, D, Jm1);
     v[0] = R.f(kLineVelX, pt);
     v[1] = R.f(kLineVelX + 1, pt);
 #pragma unroll
@@ -29,7 +29,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
     glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
     const long long gp = S.gpitch;
     const int gridY = c.gridY;
-    const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jp, ud, [&](int col, int row, double dm, double dpx, double dpy) {
+    const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jm1, ud, [&](int col, int row, double dm, double dpx, double dpy) {
         const long long gi = (long long)col * gridY + row;
         twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
         twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
@@ -51,17 +51,17 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     if (ud & (unsigned)kDisabled) return;
     const ConstsF cf = to_float(c);
     unsigned xq, yq;
-    float D[4], Jp, v[2], C[4], pp, qq;
-    load_state(R, pt, xq, yq, D, Jp);
+    float D[4], Jm1, v[2], C[4], pp, qq;
+    load_state(R, pt, xq, yq, D, Jm1);
     unsigned e;
     StressTerms terms;
     if (!g2p_point(c, cf, (const glb_f *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, 0, 0, 0, 0, xq,
-                   yq, v, C, D, Jp, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step, terms)) {
+                   yq, v, C, D, Jm1, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step, terms)) {
         atomicMax(&a.err[scene], e);
         return;
     }
     const bool rec = record >= 0 ? record != 0 : (a.clocks[scene].step % c.update_every) == 0;
-    store_state(R, pt, xq, yq, v, D, C, Jp, ud, rec, pp, qq);
+    store_state(R, pt, xq, yq, v, D, C, Jm1, ud, rec, pp, qq);
 }
 
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s)

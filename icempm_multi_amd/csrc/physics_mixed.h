@@ -88,9 +88,11 @@ __device__ __forceinline__ void kirchhoff(const ConstsF &c, const float D[4], fl
     kirchhoff(st, tau);
 }
 
-__device__ __forceinline__ void water(const ConstsF &c, float J, float tau[4])
+// Water (kernels.cuh:37-44) from q = J - 1: 1 - J^-5 = -expm1(-5 log1p(q)),
+// which keeps its relative precision for the small q of a liquid point.
+__device__ __forceinline__ void water(const ConstsF &c, float q, float tau[4])
 {
-    const float k = c.kappa * (1.f - __powf(J, -5.f));
+    const float k = -c.kappa * expm1f(-5.f * log1pf(q));
     tau[0] = k; tau[1] = 0; tau[2] = 0; tau[3] = k;
 }
 

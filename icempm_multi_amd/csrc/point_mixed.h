@@ -1,7 +1,9 @@
 // point_mixed.h -- per-point G2P and P2G of the mixed-precision form.
 //
 // Point SoA: 16 lines of 4-byte elements: 0 utility (low 32 bits), 1 P, 2 Q,
-// 3 Jp_inv, 4-5 fixed-point x/y, 6-7 v, 8-11 D = F - I, 12-15 C (FP32).
+// 3 Jp_inv - 1, 4-5 fixed-point x/y, 6-7 v, 8-11 D = F - I, 12-15 C (FP32).
+// Jp_inv and F are stored as their difference from 1 and I: a liquid point's
+// Jp_inv moves by about dt * div v per substep, a few FP32 ulps of 1.
 // Grid buffer: 6 FP32 lines (m, px, py as compensated hi/lo pairs); after the
 // grid update lines 2 and 4 hold the node velocity.
 #pragma once
@@ -93,13 +95,15 @@ __device__ __forceinline__ unsigned stencil_error(const SceneConsts &c, int bxc,
 // Drucker-Prager projection in FP64.
 // The stress terms of the updated D are left in st (st.valid false when the
 // projection changed D or the point is liquid).
-__device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF &cf, float D[4], float &Jp,
+__device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF &cf, float D[4], float &Jm1,
                                              unsigned &ud, const float A[4], float &p_out, float &q_out,
                                              StressTerms &st)
 {
     st.valid = false;
     if (ud & (unsigned)kLiquid) {
-        Jp *= 1.f + (A[0] + A[3]) + (A[0] * A[3] - A[2] * A[1]);
+        // Jp_inv *= det(I + A): (1 + q)(1 + a) - 1 = q + a + q a, a = tr A + det A
+        const float a = (A[0] + A[3]) + (A[0] * A[3] - A[2] * A[1]);
+        Jm1 = Jm1 + a + Jm1 * a;
         p_out = 0; q_out = 0;
         return;
     }
@@ -112,14 +116,14 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
     if (ud & (unsigned)kCrushed) {
         st.valid = false;
         double F[4] = {1.0 + (double)D[0], (double)D[1], (double)D[2], 1.0 + (double)D[3]};
-        double Jpd = Jp, Je, pd, qd;
+        double Jpd = 1.0 + (double)Jm1, Je, pd, qd;
         fp64::compute_pq(c, F, Je, pd, qd);
         fp64::drucker_prager(c, F, Jpd, Je, pd, qd);
         D[0] = (float)(F[0] - 1.0);
         D[1] = (float)F[1];
         D[2] = (float)F[2];
         D[3] = (float)(F[3] - 1.0);
-        Jp = (float)Jpd;
+        Jm1 = (float)(Jpd - 1.0);
         p_out = (float)pd;
         q_out = (float)qd;
     }
@@ -130,7 +134,7 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_f *g, long long gp,
                                           const lds_f *vw, int wc0, int wr0, int win_w,
                                           int win_h, unsigned &xq, unsigned &yq, float v[2], float C[4], float D[4],
-                                          float &Jp, unsigned &ud, float &p_out, float &q_out, unsigned &e,
+                                          float &Jm1, unsigned &ud, float &p_out, float &q_out, unsigned &e,
                                           unsigned point, unsigned step, StressTerms &st)
 {
     int bx, by;
@@ -211,7 +215,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     const float n2 = D[2] + A[2] + (A[2] * D[0] + A[3] * D[2]);
     const float n3 = D[3] + A[3] + (A[2] * D[1] + A[3] * D[3]);
     D[0] = n0; D[1] = n1; D[2] = n2; D[3] = n3;
-    constitutive(c, cf, D, Jp, ud, A, p_out, q_out, st);
+    constitutive(c, cf, D, Jm1, ud, A, p_out, q_out, st);
     return true;
 }
 
@@ -245,13 +249,13 @@ struct P2GStencil {
 // Stress, base node and weights of a point; returns the error byte (0 = ok).
 // terms: the stress terms of D if the G2P of this launch left them valid.
 __device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
-                                                const float v[2], const float C[4], const float D[4], float Jp,
+                                                const float v[2], const float C[4], const float D[4], float Jm1,
                                                 unsigned ud, const StressTerms &terms, P2GStencil &st)
 {
     const bool liquid = (ud & (unsigned)kLiquid) != 0;
     st.mass = liquid ? cf.wmass : cf.pmass;
     float tau[4];
-    if (liquid) water(cf, Jp, tau);
+    if (liquid) water(cf, Jm1, tau);
     else if (terms.valid) kirchhoff(terms, tau);
     else kirchhoff(cf, D, tau);
 #pragma unroll
@@ -272,30 +276,30 @@ __device__ __forceinline__ unsigned p2g_stencil(const SceneConsts &c, const Cons
 // P2G contributions of one point; calls sink(col, row, dm, dpx, dpy) (FP64 terms).
 template <class Sink>
 __device__ __forceinline__ unsigned p2g_point(const SceneConsts &c, const ConstsF &cf, unsigned xq, unsigned yq,
-                                              const float v[2], const float C[4], const float D[4], float Jp,
+                                              const float v[2], const float C[4], const float D[4], float Jm1,
                                               unsigned ud, Sink &&sink)
 {
     P2GStencil st;
     StressTerms none;
     none.valid = false;
-    const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jp, ud, none, st);
+    const unsigned e = p2g_stencil(c, cf, xq, yq, v, C, D, Jm1, ud, none, st);
     if (e) return e;
     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) { sink(st.bxc + i, st.by + j, dm, dpx, dpy); });
     return 0;
 }
 
 __device__ __forceinline__ void load_state(const PtRef &R, long long pt, unsigned &xq, unsigned &yq, float D[4],
-                                           float &Jp)
+                                           float &Jm1)
 {
     xq = R.u(kLinePosX, pt);
     yq = R.u(kLinePosX + 1, pt);
 #pragma unroll
     for (int k = 0; k < 4; k++) D[k] = R.f(kLineFe00 + k, pt);
-    Jp = R.f(kLineJp, pt);
+    Jm1 = R.f(kLineJp, pt);
 }
 
 __device__ __forceinline__ void store_state(const PtRef &R, long long pt, unsigned xq, unsigned yq, const float v[2],
-                                            const float D[4], const float C[4], float Jp, unsigned ud, bool record,
+                                            const float D[4], const float C[4], float Jm1, unsigned ud, bool record,
                                             float p, float q)
 {
     R.u(kLinePosX, pt) = xq;
@@ -307,7 +311,7 @@ __device__ __forceinline__ void store_state(const PtRef &R, long long pt, unsign
         R.setf(kLineFe00 + k, pt, D[k]);
         R.setf(kLineBp00 + k, pt, C[k]);
     }
-    R.setf(kLineJp, pt, Jp);
+    R.setf(kLineJp, pt, Jm1);
     R.u(kLineUtility, pt) = ud;
     if (record) {
         R.setf(kLineP, pt, p);
@@ -318,7 +322,7 @@ __device__ __forceinline__ void store_state(const PtRef &R, long long pt, unsign
 // State stores of the fused launch (see the FP64 form): x, y and D; Jp_inv
 // and the utility word of crushed and liquid points; P, Q on recording substeps.
 __device__ __forceinline__ void store_state_fused(const PtRef &R, long long pt, unsigned xq, unsigned yq,
-                                                  const float D[4], float Jp, unsigned ud, bool record, float p,
+                                                  const float D[4], float Jm1, unsigned ud, bool record, float p,
                                                   float q)
 {
     R.u(kLinePosX, pt) = xq;
@@ -326,7 +330,7 @@ __device__ __forceinline__ void store_state_fused(const PtRef &R, long long pt, 
 #pragma unroll
     for (int k = 0; k < 4; k++) R.setf(kLineFe00 + k, pt, D[k]);
     if (ud & (unsigned)(kLiquid | kCrushed)) {
-        R.setf(kLineJp, pt, Jp);
+        R.setf(kLineJp, pt, Jm1);
         R.u(kLineUtility, pt) = ud;
     }
     if (record) {

@@ -13,7 +13,7 @@ import pytest
 import _oracle as O
 import icempm_multi_amd as im
 from icempm_multi_amd import scenes
-from icempm_multi_amd.params import STATUS_CRUSHED, STATUS_DISABLED
+from icempm_multi_amd.params import STATUS_CRUSHED, STATUS_DISABLED, STATUS_LIQUID
 
 pytestmark = pytest.mark.gpu
 
@@ -130,6 +130,44 @@ def test_substeps_match_oracle(flags, sort):
     assert_close(download(part, h), ref, n, rtol=1e-9, what=f"flags={flags}")
     tg, sg = part.clock
     assert sg == 20 and tg == t
+
+
+def liquid_and_crushed(h, n, seed):
+    """Marks the lowest 30% of the points liquid and a fifth of the rest crushed,
+    with Jp_inv on both sides of 1 so that both branches of the return map run."""
+    r = np.random.default_rng(seed)
+    ud = h[0, :n].view(np.int64)
+    liq = h[5, :n] < np.quantile(h[5, :n], 0.3)
+    cr = ~liq & (r.random(n) < 0.2)
+    ud[liq] |= STATUS_LIQUID
+    ud[cr] |= STATUS_CRUSHED
+    h[3, :n] = np.where(cr, 0.9 + 0.2 * r.random(n), 1.0)
+    return liq, cr
+
+
+@pytest.mark.parametrize("flags", [0, im.STEP_GRAPH, im.STEP_UNFUSED, im.STEP_DIRECT_P2G])
+def test_liquid_and_crushed_points_match_oracle(flags):
+    """Liquid points (water mass and stress, Jp_inv *= det(I + dt C);
+    kernels.cuh:37-44, 61-63, 380) and points crushed from the start (the
+    Drucker-Prager return map every substep) against the oracle."""
+    P, h, n = small_block(seed=0x1CE0202)
+    with_motion(P, h, n, seed=9)
+    liq, cr = liquid_and_crushed(h, n, seed=5)
+    assert liq.sum() > 500 and cr.sum() > 200
+    ref = h.copy()
+    part = make_partition(P, h, n)
+    if not flags & im.STEP_DIRECT_P2G:
+        part.sort_points()
+        O.sort(P, ref, n)
+    part.step(7, flags)
+    part.step(13, flags)
+    t, step, _, err = O.substeps(P, ref, n, 20, 0.0, 0)
+    assert err == 0 and part.error_code == 0
+    out = download(part, h)
+    assert_close(out, ref, n, rtol=1e-9, what=f"liquid+crushed flags={flags}")
+    assert np.array_equal(out[0, :n].view(np.int64), ref[0, :n].view(np.int64))
+    liq_now = (ref[0, :n].view(np.int64) & STATUS_LIQUID) != 0
+    assert np.abs(ref[3, :n][liq_now] - 1.0).max() > 1e-9      # Jp_inv of liquid points moved
 
 
 @pytest.mark.parametrize("graph", [0, im.STEP_GRAPH])

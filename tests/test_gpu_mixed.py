@@ -12,7 +12,7 @@ import pytest
 import _oracle as O
 import icempm_multi_amd as im
 from icempm_multi_amd import scenes
-from icempm_multi_amd.params import STATUS_CRUSHED, STATUS_DISABLED
+from icempm_multi_amd.params import STATUS_CRUSHED, STATUS_DISABLED, STATUS_LIQUID
 
 pytestmark = pytest.mark.gpu
 
@@ -170,6 +170,48 @@ def test_block_2000_substeps_per_field_errors():
     assert e["rel_pos"] <= 1e-4
     assert e["rel_disp"] <= 5e-2 and e["rel_v"] <= 5e-2 and e["rel_F_minus_I"] <= 5e-2
     assert e["crushed_mismatch"] <= max(5, 0.01 * n)
+
+
+@pytest.mark.parametrize("nsub", [20, 500])
+@pytest.mark.parametrize("flags", [im.STEP_GRAPH, im.STEP_DIRECT_P2G])
+def test_mixed_liquid_and_crushed_points(flags, nsub):
+    """The mixed form on a block whose lowest 30% is liquid and a fifth of the
+    rest crushed from the start, against the FP64 oracle.  A liquid point's
+    Jp_inv moves by about dt * div v per substep, a few FP32 ulps of 1: the
+    form stores Jp_inv - 1, and the bounds below fail by orders of magnitude
+    when it is stored as Jp_inv (errors of 50% in v and in Jp_inv - 1 after 20
+    substeps)."""
+    P, h, n = scenes.block_scene(nx=60, ny=16, grid=(96, 48), n_grains=40, seed=0x1CE0910)
+    r = np.random.default_rng(4)
+    ud = h[0, :n].view(np.int64)
+    liq = h[5, :n] < np.quantile(h[5, :n], 0.3)
+    cr = ~liq & (r.random(n) < 0.2)
+    ud[liq] |= STATUS_LIQUID
+    ud[cr] |= STATUS_CRUSHED
+    h[3, :n] = np.where(cr, 0.9 + 0.2 * r.random(n), 1.0)
+    ref = h.copy()
+    part = make(P, h, n, im.PRECISION_MIXED)
+    if not flags & im.STEP_DIRECT_P2G:
+        part.sort_points()
+        O.sort(P, ref, n)
+    x0 = ref[4:6, :n].copy()
+    part.step(nsub, flags)
+    O.substeps(P, ref, n, nsub, 0.0, 0)
+    out = download(part, h)
+    assert part.error_code == 0
+    e = errors(out, ref, x0, n)
+    lo = (out[0, :n].view(np.int64) & STATUS_LIQUID) != 0
+    lr = (ref[0, :n].view(np.int64) & STATUS_LIQUID) != 0
+    e["rel_Jp_liquid"] = np.linalg.norm(out[3, :n][lr] - ref[3, :n][lr]) / np.linalg.norm(ref[3, :n][lr] - 1.0)
+    e["rel_v_liquid"] = np.linalg.norm(out[6:8, :n][:, lr] - ref[6:8, :n][:, lr]) / np.linalg.norm(ref[6:8, :n][:, lr])
+    e["rel_v_ice"] = np.linalg.norm(out[6:8, :n][:, ~lr] - ref[6:8, :n][:, ~lr]) / np.linalg.norm(ref[6:8, :n][:, ~lr])
+    print(f"MIXLIQ nsub={nsub} flags={flags}:", {k: float(f"{v:.3g}") for k, v in e.items()})
+    assert e["rel_pos"] <= 1e-6
+    assert e["rel_v"] <= 1e-4 and e["rel_v_liquid"] <= 1e-4
+    assert e["rel_F_minus_I"] <= 1e-4
+    assert e["rel_Jp_liquid"] <= 1e-4
+    assert e["crushed_mismatch"] == 0
+    assert int(np.count_nonzero(lo)) == int(liq.sum())
 
 
 def test_mixed_error_code():
