@@ -170,6 +170,37 @@ def test_liquid_and_crushed_points_match_oracle(flags):
     assert np.abs(ref[3, :n][liq_now] - 1.0).max() > 1e-9      # Jp_inv of liquid points moved
 
 
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_walls_and_vmax_clamp_match_oracle(side):
+    """The grid update's side-wall rules (kernels.cuh:290-294) with the block
+    pushed against the left or right wall, and the vmax clamp (261) on a
+    cluster moving at 3e4 m/s (vmax = 0.5 h / dt = 12,500 m/s here)."""
+    P, h, n = small_block(seed=0x1CE0203)
+    x = h[4, :n]
+    if side == "left":
+        x -= x.min() - 0.55          # first column of points within the wall nodes
+        h[6, :n] = -0.5
+    else:
+        x += (P.GridXTotal * P.cellsize - 0.55) - x.max()
+        h[6, :n] = 0.5
+    c = np.array([x.mean(), h[5, :n].mean()])
+    fast = np.hypot(x - c[0], h[5, :n] - c[1]) < 1.0
+    assert fast.sum() > 20
+    h[6, :n][fast] = 3e4
+    ref = h.copy()
+    part = make_partition(P, h, n)
+    part.sort_points()
+    O.sort(P, ref, n)
+    part.step(3, im.STEP_GRAPH)
+    t, step, _, err = O.substeps(P, ref, n, 3, 0.0, 0)
+    assert err == 0 and part.error_code == 0
+    out = download(part, h)
+    assert_close(out, ref, n, rtol=1e-8, what=f"walls/vmax {side}")
+    # the clamp acted: the fast points' speed is bounded by vmax after G2P
+    v = np.hypot(ref[6, :n], ref[7, :n])
+    assert v.max() <= P.vmax * (1 + 1e-12) and v.max() > 0.5 * P.vmax
+
+
 @pytest.mark.parametrize("graph", [0, im.STEP_GRAPH])
 def test_sampled_timings_count_launches(graph):
     """Events around the launches of substeps 0, 32, 64 of a 70-substep call:
