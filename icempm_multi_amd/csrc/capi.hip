@@ -5,6 +5,7 @@
 // for icempm_last_error() on the calling thread.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -399,9 +400,9 @@ int icempm_batch_add_scene(icempm_batch *b, const icempm_simparams *prms, size_t
         icempm::set_last_error("add_scene after commit");
         return ICEMPM_ERR_STATE;
     }
-    if (n_points <= 0 || (unsigned)n_points > hssoa_capacity) return bad("add_scene: bad point count");
+    if (n_points < 0 || (unsigned)n_points > hssoa_capacity) return bad("add_scene: bad point count");
     return guarded([&] {
-        const int id = b->eng->add_scene(prms, n_points);
+        const int id = b->eng->add_scene(prms, std::max(n_points, 1));   // an empty scene keeps a 1-point range
         if (id < 0) return id;
         std::vector<double> copy((size_t)ICEMPM_N_PTS_ARRAYS * n_points);
         for (int line = 0; line < ICEMPM_N_PTS_ARRAYS; line++)

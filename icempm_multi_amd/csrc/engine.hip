@@ -517,7 +517,7 @@ int Engine::upload(int scene, const double *hssoa, unsigned capacity, int offset
     if (offset < 0 || (unsigned)(offset + n) > capacity)
         return fail(ICEMPM_ERR_CAPACITY, "transfer_points_from_soa_to_device: HSSOA capacity");
     if (precision_ == ICEMPM_PRECISION_FP64) {
-        for (int line = 0; line < kNumPtsLines; line++)
+        for (int line = 0; line < kNumPtsLines && n > 0; line++)
             HIPCHK(hipMemcpyAsync((double *)pts_[0] + line * pitch_ + s.pt_off, hssoa + (size_t)line * capacity + offset,
                                   sizeof(double) * n, hipMemcpyHostToDevice, stream_));
     } else {
@@ -551,7 +551,7 @@ int Engine::upload(int scene, const double *hssoa, unsigned capacity, int offset
             putf(kLineFe00 + 3, L(kLineFe00 + 3, i) - 1.0);
             for (int k = 0; k < 4; k++) putf(kLineBp00 + k, L(kLineBp00 + k, i));
         }
-        for (int line = 0; line < kNumPtsLines; line++)
+        for (int line = 0; line < kNumPtsLines && n > 0; line++)
             HIPCHK(hipMemcpyAsync((uint32_t *)pts_[0] + line * pitch_ + s.pt_off, buf.data() + (size_t)line * n,
                                   sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
@@ -572,13 +572,13 @@ int Engine::download(int scene, double *hssoa, unsigned capacity, int offset, in
     if (offset < 0 || (unsigned)(offset + n) > capacity)
         return fail(ICEMPM_ERR_CAPACITY, "transfer_from_device() HSSOA capacity");
     if (precision_ == ICEMPM_PRECISION_FP64) {
-        for (int line = 0; line < kNumPtsLines; line++)
+        for (int line = 0; line < kNumPtsLines && n > 0; line++)
             HIPCHK(hipMemcpyAsync(hssoa + (size_t)line * capacity + offset, (double *)pts_[0] + line * pitch_ + s.pt_off,
                                   sizeof(double) * n, hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
     } else {
         std::vector<uint32_t> buf((size_t)kNumPtsLines * std::max(n, 1));
-        for (int line = 0; line < kNumPtsLines; line++)
+        for (int line = 0; line < kNumPtsLines && n > 0; line++)
             HIPCHK(hipMemcpyAsync(buf.data() + (size_t)line * n, (uint32_t *)pts_[0] + line * pitch_ + s.pt_off,
                                   sizeof(uint32_t) * n, hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));

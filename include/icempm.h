@@ -269,7 +269,8 @@ typedef struct icempm_batch icempm_batch;
 
 int icempm_batch_create(int device, int precision, icempm_batch **out);
 int icempm_batch_destroy(icempm_batch *b);
-/* Adds a scene: its parameters and n_points points from a HostSideSOA buffer. */
+/* Adds a scene: its parameters and n_points points from a HostSideSOA buffer
+ * (n_points may be 0: the scene's clock still advances with the others). */
 int icempm_batch_add_scene(icempm_batch *b, const icempm_simparams *prms, size_t prms_bytes,
                            const double *hssoa, unsigned hssoa_capacity, int n_points, int *scene_id);
 /* Allocates device memory for all scenes, uploads them and bins them. */

@@ -1,0 +1,100 @@
+"""Edge cases of the point set through the C-ABI: no points, a single point,
+a count that fills no wave or tile evenly, and a batch with an empty scene.
+Stepping must run, advance the clock, and agree with the FP64 oracle."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import icempm_multi_amd as im
+from icempm_multi_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+FLAG_SETS = [0, im.STEP_GRAPH, im.STEP_UNFUSED, im.STEP_DIRECT_P2G]
+
+
+def make(P, h, n, precision, cap=None):
+    part = im.Partition(0, 0, precision)
+    part.allocate(cap or h.shape[1], P.GridXTotal)
+    part.update_constants(P)
+    part.transfer_points_from_soa_to_device(h, 0, n)
+    return part
+
+
+def moving_block(seed):
+    P, h, n = scenes.block_scene(nx=48, ny=12, grid=(96, 48), n_grains=40, seed=seed)
+    h[6, :n] = 0.4
+    h[7, :n] = -0.2 * np.sin(h[4, :n])
+    return P, h, n
+
+
+@pytest.mark.parametrize("precision", [im.PRECISION_FP64, im.PRECISION_MIXED])
+@pytest.mark.parametrize("flags", FLAG_SETS)
+def test_no_points(precision, flags):
+    P, h, n = moving_block(0x1CE0A01)
+    part = make(P, h, 0, precision, cap=64)
+    if not flags & im.STEP_DIRECT_P2G:
+        part.sort_points()
+    assert part.nPts_partition == 0
+    part.step(12, flags)
+    part.step(7, flags)
+    assert part.error_code == 0
+    t, step = part.clock
+    assert step == 19 and t > 0
+    out = np.zeros((16, 64))
+    part.transfer_from_device(out, 0)
+    assert np.all(out == 0)
+    assert np.all(part.indenter_force() == 0)
+    part.close()
+
+
+@pytest.mark.parametrize("count", [1, 2, 63, 257, 1031])
+@pytest.mark.parametrize("flags", FLAG_SETS)
+def test_few_points_match_oracle(count, flags):
+    """Counts below and between wave (64) and tile-iteration (256) multiples."""
+    P, h, n = moving_block(0x1CE0A02)
+    idx = np.random.default_rng(count).choice(n, size=count, replace=False)
+    hs = np.zeros((16, count + 8))
+    hs[:, :count] = h[:, idx]
+    P.nPtsTotal = count
+    ref = hs.copy()
+    part = make(P, hs, count, im.PRECISION_FP64)
+    if not flags & im.STEP_DIRECT_P2G:
+        part.sort_points()
+        O.sort(P, ref, count)
+    part.step(9, flags)
+    part.step(11, flags)
+    t, step, _, err = O.substeps(P, ref, count, 20, 0.0, 0)
+    assert err == 0 and part.error_code == 0
+    out = np.zeros_like(hs)
+    part.transfer_from_device(out, 0)
+    for line in range(3, 16):
+        sc = np.abs(ref[line, :count]).max() + 1e-30
+        assert np.abs(out[line, :count] - ref[line, :count]).max() <= 1e-9 * sc + 1e-13, (count, flags, line)
+    assert part.clock == (t, step)
+    part.close()
+
+
+@pytest.mark.parametrize("precision", [im.PRECISION_FP64, im.PRECISION_MIXED])
+def test_batch_with_an_empty_scene(precision):
+    P1, h1, n1 = moving_block(0x1CE0A03)
+    P2, h2, n2 = moving_block(0x1CE0A04)
+    bt = im.Batch(0, precision)
+    bt.add_scene(P1, h1, n1)
+    bt.add_scene(P2, h2, 0)
+    bt.add_scene(P2, h2, n2)
+    bt.commit()
+    bt.step(30, im.STEP_GRAPH | im.STEP_RESORT)
+    bt.step(10, 0)
+    assert bt.error_code == 0
+    ref = h1.copy()
+    O.sort(P1, ref, n1)
+    O.substeps(P1, ref, n1, 40, 0.0, 0)
+    out = np.zeros_like(h1)
+    bt.transfer_from_device(0, out)
+    tol = 1e-9 if precision == im.PRECISION_FP64 else 1e-4
+    assert np.linalg.norm(out[4:6, :n1] - ref[4:6, :n1]) <= tol * np.linalg.norm(ref[4:6, :n1])
+    empty = np.zeros_like(h2)
+    bt.transfer_from_device(1, empty)
+    assert np.all(empty == 0)
+    bt.close()
