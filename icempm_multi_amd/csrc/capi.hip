@@ -32,13 +32,7 @@ struct icempm_partition {
 
 struct icempm_batch {
     std::unique_ptr<Engine> eng;
-    struct Pending {
-        const double *hssoa;
-        unsigned capacity;
-        int n;
-    };
-    std::vector<std::vector<double>> staged;   // host copies until commit
-    std::vector<unsigned> staged_cap;
+    std::vector<std::vector<double>> staged;   // host copies until a commit succeeds
     std::vector<int> staged_n;
     bool committed = false;
 };
@@ -418,17 +412,26 @@ int icempm_batch_add_scene(icempm_batch *b, const icempm_simparams *prms, size_t
 int icempm_batch_commit(icempm_batch *b)
 {
     if (!b) return bad("null batch");
+    if (b->committed) {
+        icempm::set_last_error("batch already committed");
+        return ICEMPM_ERR_STATE;
+    }
     return guarded([&] {
+        // the host copies are released only once every scene is on the device
+        // and binned, so a failed commit can be retried
         int rc = b->eng->commit();
         if (rc) return rc;
         for (int i = 0; i < (int)b->staged.size(); i++) {
             rc = b->eng->upload(i, b->staged[i].data(), (unsigned)b->staged_n[i], 0, b->staged_n[i]);
             if (rc) return rc;
-            b->staged[i].clear();
-            b->staged[i].shrink_to_fit();
         }
+        rc = b->eng->sort();
+        if (rc) return rc;
+        b->staged.clear();
+        b->staged.shrink_to_fit();
+        b->staged_n.clear();
         b->committed = true;
-        return b->eng->sort();
+        return ICEMPM_OK;
     });
 }
 

@@ -514,7 +514,7 @@ int Engine::upload(int scene, const double *hssoa, unsigned capacity, int offset
     RC(ensure_committed());
     SceneHost &s = scenes_[scene];
     if (n < 0 || n > s.cap_pts) return fail(ICEMPM_ERR_CAPACITY, "transfer_points_from_soa_to_device: capacity");
-    if (offset < 0 || (unsigned)(offset + n) > capacity)
+    if (offset < 0 || (long long)offset + n > (long long)capacity)
         return fail(ICEMPM_ERR_CAPACITY, "transfer_points_from_soa_to_device: HSSOA capacity");
     if (precision_ == ICEMPM_PRECISION_FP64) {
         for (int line = 0; line < kNumPtsLines && n > 0; line++)
@@ -569,7 +569,7 @@ int Engine::download(int scene, double *hssoa, unsigned capacity, int offset, in
     RC(ensure_committed());
     SceneHost &s = scenes_[scene];
     const int n = s.n_pts;
-    if (offset < 0 || (unsigned)(offset + n) > capacity)
+    if (offset < 0 || (long long)offset + n > (long long)capacity)
         return fail(ICEMPM_ERR_CAPACITY, "transfer_from_device() HSSOA capacity");
     if (precision_ == ICEMPM_PRECISION_FP64) {
         for (int line = 0; line < kNumPtsLines && n > 0; line++)

@@ -89,3 +89,11 @@ STEP_TIMINGS = 0x4
 STEP_DIRECT_P2G = 0x8
 STEP_UNFUSED = 0x10
 STEP_SAMPLED = 0x20
+
+# status codes of every icempm_* call (ICEMPM_ERR_* of include/icempm.h)
+ERR_ARGUMENT = -1
+ERR_HIP = -2
+ERR_CAPACITY = -3
+ERR_STATE = -4
+ERR_NOMEM = -5
+ERR_DEVICE = -6

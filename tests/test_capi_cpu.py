@@ -31,6 +31,22 @@ def test_library_exports_every_declared_function():
     assert set(names) == set(capi.EXPORTED_SYMBOLS)
 
 
+def test_python_constants_match_header_defines():
+    # the Python names of the status codes, step flags, precisions and point
+    # status bits carry the values include/icempm.h defines
+    text = open(os.path.join(ROOT, "include", "icempm.h")).read()
+    defs = {m.group(1): int(m.group(2), 0)
+            for m in re.finditer(r"#define\s+ICEMPM_([A-Z0-9_]+)\s+\(?(-?(?:0x[0-9A-Fa-f]+|\d+))\)?", text)}
+    checked = 0
+    for name, value in defs.items():
+        if hasattr(im, name):
+            assert getattr(im, name) == value, name
+            checked += 1
+    for name in ("ERR_ARGUMENT", "ERR_STATE", "ERR_DEVICE", "STEP_GRAPH", "STEP_SAMPLED", "PRECISION_MIXED"):
+        assert name in defs and hasattr(im, name), name
+    assert checked >= 14
+
+
 def test_struct_size_matches_header_mirror():
     import ctypes
     from _oracle import simparams_offsets

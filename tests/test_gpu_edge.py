@@ -98,3 +98,25 @@ def test_batch_with_an_empty_scene(precision):
     bt.transfer_from_device(1, empty)
     assert np.all(empty == 0)
     bt.close()
+
+
+def test_batch_commit_twice_is_refused():
+    # a second commit must fail cleanly (the host copies of the scenes are
+    # released by the first) and leave the committed batch usable
+    P1, h1, n1 = moving_block(0x1CE0A05)
+    bt = im.Batch(0, im.PRECISION_FP64)
+    bt.add_scene(P1, h1, n1)
+    bt.commit()
+    with pytest.raises(im.IcempmError) as e:
+        bt.commit()
+    assert e.value.code == im.ERR_STATE and "already committed" in e.value.message
+    with pytest.raises(im.IcempmError):
+        bt.add_scene(P1, h1, n1)
+    bt.step(10, 0)
+    ref = h1.copy()
+    O.sort(P1, ref, n1)
+    O.substeps(P1, ref, n1, 10, 0.0, 0)
+    out = np.zeros_like(h1)
+    bt.transfer_from_device(0, out)
+    assert np.linalg.norm(out[4:6, :n1] - ref[4:6, :n1]) <= 1e-9 * np.linalg.norm(ref[4:6, :n1])
+    bt.close()
