@@ -789,6 +789,7 @@ int Engine::synchronize()
 
 int Engine::get_indenter_force(int scene, double *out, int n)
 {
+    if (n < 0) return fail(ICEMPM_ERR_ARGUMENT, "get_indenter_force: negative length");
     RC(ensure_committed());
     const SceneHost &s = scenes_[scene];
     const int m = std::min(n, 2 * s.prms.n_indenter_subdivisions);
@@ -883,6 +884,7 @@ int Engine::sort()
 
 int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts, int n_cells)
 {
+    if (n < 0 || n_cells < 0) return fail(ICEMPM_ERR_ARGUMENT, "get_bins: negative length");
     if (!bins_valid_) return fail(ICEMPM_ERR_STATE, "get_bins: no sort since the last upload");
     const SceneHost &s = scenes_[scene];
     const int m = std::min(n, s.n_pts);

@@ -90,7 +90,8 @@ STEP_DIRECT_P2G = 0x8
 STEP_UNFUSED = 0x10
 STEP_SAMPLED = 0x20
 
-# status codes of every icempm_* call (ICEMPM_ERR_* of include/icempm.h)
+# status codes of every icempm_* call (ICEMPM_OK, ICEMPM_ERR_* of include/icempm.h)
+OK = 0
 ERR_ARGUMENT = -1
 ERR_HIP = -2
 ERR_CAPACITY = -3

@@ -120,3 +120,22 @@ def test_batch_commit_twice_is_refused():
     bt.transfer_from_device(0, out)
     assert np.linalg.norm(out[4:6, :n1] - ref[4:6, :n1]) <= 1e-9 * np.linalg.norm(ref[4:6, :n1])
     bt.close()
+
+
+def test_negative_lengths_are_argument_errors():
+    import ctypes as C
+    P, h, n = moving_block(0x1CE0A06)
+    part = make(P, h, n, im.PRECISION_FP64)
+    part.sort_points()
+    L = im.lib()
+    buf = np.zeros(16)
+    ibuf = np.zeros(16, dtype=np.int32)
+    d = buf.ctypes.data_as(C.POINTER(C.c_double))
+    i = ibuf.ctypes.data_as(C.POINTER(C.c_int32))
+    assert L.icempm_partition_get_indenter_force(part._h, d, -1) == im.ERR_ARGUMENT
+    assert L.icempm_partition_get_bins(part._h, i, None, -1, None, 0) == im.ERR_ARGUMENT
+    assert L.icempm_partition_get_bins(part._h, None, None, 0, i, -5) == im.ERR_ARGUMENT
+    assert L.icempm_partition_get_bins(part._h, i, i, 16, None, 0) == im.OK
+    part.step(3, 0)
+    assert part.error_code == 0
+    part.close()
