@@ -443,6 +443,31 @@ def test_substeps_free_fall_momentum_and_mass():
     assert abs(soa[6, :n].mean()) < 1e-12
 
 
+@pytest.mark.parametrize("comp", [0, 1], ids=["P", "S"])
+def test_elastic_wave_speeds(comp):
+    # P and S fronts run at the speeds the linearised stress fixes,
+    # sqrt((mu + kappa)/rho) and sqrt(mu/rho), less MPM's ~1.3% grid dispersion;
+    # the three-dimensional P modulus lambda + 2 mu would read 4% fast, a
+    # mu/kappa swap 47% fast on S, a factor 4 in the P2G force scale 2x off
+    import _waves as W
+    P, hs, n = W.block()
+    O.sort(P, hs, n)
+    probes, nsub = W.start(P, hs, n, comp)
+    t, step = 0.0, 0
+    times, series = [], [[] for _ in probes]
+    for _ in range(nsub // W.CHUNK):
+        t, step, _f, err = O.substeps(P, hs, n, W.CHUNK, t, step)
+        assert err == 0
+        times.append(t)
+        for s_, m in zip(series, probes):
+            s_.append(hs[6 + comp, :n][m].mean())
+    ratio = W.measured_speed(times, series) / W.expected_speed(P, comp)
+    assert 0.97 <= ratio <= 1.01, ratio
+    # behind the front both probes move at V/2: equal impedance on both sides
+    for s_ in series:
+        assert abs(s_[-1] - W.V / 2) <= 0.1 * W.V / 2, s_[-1]
+
+
 def test_cell_key_and_stable_sort():
     P = tiny_params(GX=64, GY=40)
     n = 3000
