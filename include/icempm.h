@@ -197,7 +197,9 @@ int icempm_partition_synchronize(icempm_partition *p);
 int icempm_partition_get_indenter_force(icempm_partition *p, double *out, int n_doubles);
 /* The error byte (gpu_error_indicator), read back after a sync; 0 = none. */
 int icempm_partition_get_error_code(icempm_partition *p, uint8_t *code);
-/* nPts_partition and the number of disabled points (gpu_partition.h:122-123). */
+/* nPts_partition and nPts_disabled (gpu_partition.h:122-123).  The reference
+ * counts in nPts_disabled the points handed to another partition
+ * (gpu_implementation5.cpp:146), so with one partition it is always 0. */
 int icempm_partition_get_point_count(icempm_partition *p, int *n_points, int *n_disabled);
 /* GPU_Partition timing_* fields (gpu_partition.h:156-162) in ms summed since the
  * last reset_indenter_force_accumulator: out[0]=P2G, [1]=halo (0), [2]=update,
