@@ -155,11 +155,12 @@ int icempm_partition_allocate(icempm_partition *p, int n_points_capacity, int gr
 {
     if (!p) return bad("null partition");
     if (n_points_capacity <= 0) return bad("allocate: capacity must be positive");
+    if (p->allocated || p->eng->nscenes() != 0) return bad("allocate: already allocated");
     return guarded([&] {
+        const int id = p->eng->add_scene(nullptr, n_points_capacity);
+        if (id < 0) return id;
         p->cap = n_points_capacity;
         p->grid_x_cap = grid_x_capacity;
-        if (p->eng->nscenes() == 0) p->eng->add_scene(nullptr, n_points_capacity);
-        else return bad("allocate: already allocated");
         p->allocated = true;
         return ICEMPM_OK;
     });
