@@ -128,6 +128,8 @@ private:
     int *d_dirty_ = nullptr;   // 2 * n_segs_ segment flags
     int clock_slot_ = 0;       // slot of d_clocks_ (2 x nscenes) holding the current clock
     // workgroups of a grid update over n runs: a run per wave, persistent beyond 8 per CU
+    int tile_blocks(int mode);      // workgroups of a tile launch: the resident count
+    int resident_[4] = {0, 0, 0, 0};   // per tile mode; reset when the tile shape changes
     int update_blocks(int n_runs) const { return std::max(1, std::min((n_runs + 3) / 4, n_cus_ * 8)); }
     long long n_segs_ = 0;
     NodeChunk *d_chunks_ = nullptr;

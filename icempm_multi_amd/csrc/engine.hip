@@ -483,6 +483,7 @@ int Engine::set_tile_shape(int tx, int ty, int margin)
     const size_t lds = (size_t)(tx + 2 + 2 * margin) * (ty + 2 + 2 * margin) * per_node;
     if (lds > 64 * 1024) return fail(ICEMPM_ERR_ARGUMENT, "tile window exceeds the LDS budget");
     if (tx == tx_ && ty == ty_ && margin == margin_) return ICEMPM_OK;
+    for (int &r : resident_) r = 0;
     tx_ = tx;
     ty_ = ty;
     margin_ = margin;
@@ -699,14 +700,29 @@ int Engine::reset_grid(int scene)
     return rc;
 }
 
+// Workgroups of a tile launch.  The kernels loop over the tile list, and a
+// launch of exactly the workgroups the GPU holds at once (one generation) is
+// the fastest: with more, the later generations start behind the first and the
+// launch ends on a tail of partly busy CUs (4M-point sheet, fused launch: FP64
+// 139.7 us with 8 workgroups per CU, 133.5 us with 2, its residency; mixed
+// 106.0 us with 8, 103.6 us with 4, its residency, 133 us with 2).
+int Engine::tile_blocks(int mode)
+{
+    int &r = resident_[mode & 3];
+    if (r <= 0) {
+        r = LAUNCH_PREC(tiles_resident_per_cu, args(), mode);
+        if (r <= 0) r = 8;   // the runtime could not tell: several generations
+    }
+    return (int)std::min<long long>(std::max<long long>(max_tiles_, 1), (long long)n_cus_ * r);
+}
+
 int Engine::p2g(int scene)
 {
     RC(ensure_committed());
     const StepArgs a = args();
     timed_begin(true);
     if (bins_valid_ && nscenes() == 1) {
-        const int nb = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
-        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, 0, 0, 0, nb, stream_));
+        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, 0, 0, 0, tile_blocks(kTileP2G), stream_));
     } else {
         HIPCHK(LAUNCH_PREC(launch_p2g_direct, a, scene, scenes_[scene].n_pts, 0, stream_));
     }
@@ -755,8 +771,7 @@ int Engine::g2p(int scene, int recordPQ)
     const StepArgs a = args();
     timed_begin(true);
     if (bins_valid_ && nscenes() == 1) {
-        const int nb = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
-        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, 0, 0, recordPQ ? 1 : 0, nb, stream_));
+        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, 0, 0, recordPQ ? 1 : 0, tile_blocks(kTileG2P), stream_));
     } else {
         HIPCHK(LAUNCH_PREC(launch_g2p_direct, a, scene, scenes_[scene].n_pts, 0, recordPQ ? 1 : 0, stream_));
     }
@@ -912,7 +927,6 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
     StepArgs a = args();
     SceneClock *slot[2] = {d_clocks_, d_clocks_ + nscenes()};
     int cs = clock_slot_;
-    const int nb_tiles = (int)std::min<long long>(std::max<long long>(max_tiles_, 1), n_cus_ * 8LL);
     const int nb_upd = update_blocks(n_chunks_);
     int b = start_buf;
     for (int k = 0; k < nsub; k++) {
@@ -949,7 +963,7 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
         }
         if (k == 0 || !fused) {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, nb_tiles, stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, tile_blocks(kTileP2G), stream_));
             timed_end(timed, kClsP2G);
         }
         timed_begin(timed);
@@ -957,11 +971,11 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
         timed_end(timed, kClsUpdate);
         if (fused && k < nsub - 1) {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileFused, b, b ^ 1, -1, nb_tiles, stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileFused, b, b ^ 1, -1, tile_blocks(kTileFused), stream_));
             timed_end(timed, kClsFused);
         } else {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, -1, nb_tiles, stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, -1, tile_blocks(kTileG2P), stream_));
             timed_end(timed, kClsG2P);
         }
         b ^= 1;
@@ -1003,6 +1017,10 @@ int Engine::step(int nsub, int flags)
             // The capture records the timing events into the graph: it takes
             // them from the pool (emptied first) and keeps them.
             if (timed) RC(collect_timings());
+            // workgroup counts are runtime queries: make them before the capture
+            tile_blocks(kTileG2P);
+            tile_blocks(kTileP2G);
+            tile_blocks(kTileFused);
             HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             capturing_ = true;
             const int rc = launch_substeps(nsub, start_buf, fused, direct, timed, sample);

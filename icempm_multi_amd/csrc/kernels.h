@@ -18,6 +18,9 @@ hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, 
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
+// Workgroups of the tile kernel for this mode and window that one CU holds at
+// once (registers and LDS); 0 if the runtime cannot tell.
+int tiles_resident_per_cu(const StepArgs &a, int mode);
 }  // namespace fp64
 
 namespace mixed {
@@ -28,6 +31,9 @@ hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, 
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
+// Workgroups of the tile kernel for this mode and window that one CU holds at
+// once (registers and LDS); 0 if the runtime cannot tell.
+int tiles_resident_per_cu(const StepArgs &a, int mode);
 }  // namespace mixed
 
 // kernels_common.hip

@@ -315,5 +315,30 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int rec
     }
 }
 
+template <int MODE>
+static int resident_wh(const StepArgs &a, size_t lds)
+{
+    int n = 0;
+    hipError_t e;
+    switch (a.win_h) {
+    case 20: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 20>, kBlock, lds); break;
+    case 36: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 36>, kBlock, lds); break;
+    case 68: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 68>, kBlock, lds); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 0>, kBlock, lds); break;
+    }
+    return e == hipSuccess ? n : 0;
+}
+
+int tiles_resident_per_cu(const StepArgs &a, int mode)
+{
+    const size_t lds = tile_lds_bytes(a, mode);
+    switch (mode) {
+    case kTileG2P: return resident_wh<kTileG2P>(a, lds);
+    case kTileP2G: return resident_wh<kTileP2G>(a, lds);
+    case kTileFused: return resident_wh<kTileFused>(a, lds);
+    default: return 0;
+    }
+}
+
 }  // namespace fp64
 }  // namespace icempm
