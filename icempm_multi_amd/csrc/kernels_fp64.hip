@@ -28,8 +28,8 @@ namespace fp64 {
 
 // ---------------------------------------------------------------------------
 // Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
-// 2 waves per SIMD (up to 256 VGPRs; the fused kernel needs about 236): at 3
-// waves (168 VGPRs) it spills about 36 VGPRs in the point loop, and a spill
+// 2 waves per SIMD (up to 256 VGPRs; the fused kernel needs about 242): at 3
+// waves (168 VGPRs) it spills 36-64 VGPRs in the point loop, and a spill
 // reload waits for every outstanding load, the prefetch of the next point
 // included (measured slower on the 4M-point sheet).
 // ---------------------------------------------------------------------------
