@@ -184,7 +184,7 @@ def batch_workload(args, im, scenes):
             bt.add_scene(P, h, n)
         bt.commit()
         chosen, ms = bt.autotune(2)
-        log(f"[{args.workload} {name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in ms[:15]]}")
+        log(f"[{args.workload} {name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in ms[:28]]}")
         bt.reset_kernel_stats()
         bt.step(20, im.STEP_TIMINGS)
         st = bt.kernel_stats()
@@ -270,7 +270,7 @@ def main():
             chosen = args.tile
         else:
             chosen, tune_ms = part.autotune(3)
-            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:15]]}")
+            log(f"[{precision_name}] autotune: candidate {chosen}, ms/substep {['%.3f' % m for m in tune_ms[:28]]}")
         frame = P.UpdateEveryNthStep
         flags_timed = (im.STEP_GRAPH | im.STEP_TIMINGS | im.STEP_SAMPLED) if args.mode == "graph" else im.STEP_TIMINGS
 

@@ -288,7 +288,7 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
     return n * sizeof(double);
 }
 
-// window heights compiled in: ty = 16, 32, 64 with a drift margin of 1
+// window heights compiled in: ty = 16, 32, 64 with a drift margin of 1 or 0
 template <int MODE>
 static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int record_mode, int n_blocks, size_t lds,
                                   hipStream_t s)
@@ -297,6 +297,9 @@ static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int rec
     case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
     case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
     case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 18: hipLaunchKernelGGL((k_tiles<MODE, 18>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 34: hipLaunchKernelGGL((k_tiles<MODE, 34>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 66: hipLaunchKernelGGL((k_tiles<MODE, 66>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
     default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
     }
     return hipGetLastError();
@@ -324,6 +327,9 @@ static int resident_wh(const StepArgs &a, size_t lds)
     case 20: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 20>, kBlock, lds); break;
     case 36: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 36>, kBlock, lds); break;
     case 68: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 68>, kBlock, lds); break;
+    case 18: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 18>, kBlock, lds); break;
+    case 34: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 34>, kBlock, lds); break;
+    case 66: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 66>, kBlock, lds); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<MODE, 0>, kBlock, lds); break;
     }
     return e == hipSuccess ? n : 0;

@@ -259,8 +259,10 @@ int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread);
 int icempm_partition_set_tile_schedule(icempm_partition *p, int dynamic);
 
 /* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
- * the current point set and keeps the fastest.  Results (ms per launch) of each
- * candidate go to out_ms if non-null. */
+ * the current point set, each with the current window drift margin and with
+ * the other one (1 or 0), and keeps the fastest.  Results (ms per launch; -1
+ * for a shape that does not fit) go to out_ms if non-null: the candidates at
+ * the current margin, then at the other.  *chosen = the index kept. */
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen);
 
 /* ------------------------------------------------------------------------- */
