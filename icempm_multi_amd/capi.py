@@ -69,6 +69,9 @@ _SIGS = {
     "icempm_partition_set_lane_map": (C.c_int, [_VP, C.c_int]),
     "icempm_partition_set_tile_schedule": (C.c_int, [_VP, C.c_int]),
     "icempm_partition_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    "icempm_partition_get_tile_config": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                   C.POINTER(C.c_int)]),
+    "icempm_partition_get_graph_stats": (C.c_int, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
     "icempm_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_VP)]),
     "icempm_batch_destroy": (C.c_int, [_VP]),
     "icempm_batch_add_scene": (C.c_int, [_VP, _P, C.c_size_t, _D, C.c_uint, C.c_int, C.POINTER(C.c_int)]),
@@ -77,6 +80,12 @@ _SIGS = {
     "icempm_batch_transfer_from_device": (C.c_int, [_VP, C.c_int, _D, C.c_uint]),
     "icempm_batch_get_indenter_force": (C.c_int, [_VP, C.c_int, _D, C.c_int]),
     "icempm_batch_get_error_code": (C.c_int, [_VP, C.POINTER(C.c_uint8)]),
+    "icempm_batch_get_scene_point_count": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int)]),
+    "icempm_batch_get_bins": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int,
+                                        C.POINTER(C.c_int32), C.c_int]),
+    "icempm_batch_get_grid": (C.c_int, [_VP, C.c_int, C.c_int, _D, C.c_longlong]),
+    "icempm_batch_get_position_map": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                C.POINTER(C.c_int)]),
     "icempm_batch_get_scene_count": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "icempm_batch_last_step_ms": (C.c_int, [_VP, C.POINTER(C.c_float)]),
     "icempm_batch_get_kernel_stats": (C.c_int, [_VP, _D, C.POINTER(C.c_longlong), C.c_int]),
@@ -84,6 +93,9 @@ _SIGS = {
     "icempm_poisson_disk": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, _D, _D,
                                       C.c_longlong, C.POINTER(C.c_longlong)]),
     "icempm_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    "icempm_batch_get_tile_config": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int)]),
+    "icempm_batch_get_graph_stats": (C.c_int, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
@@ -255,10 +267,24 @@ class Partition:
         _check(lib().icempm_partition_set_tile_schedule(self._h, dynamic))
 
     def autotune(self, n_trials=5):
+        """Returns (chosen index, ms per fused launch of the 14 candidates at
+        margin 0 then margin 1); tile_config() gives the shape kept."""
         out = (C.c_float * 32)()
         chosen = C.c_int(-1)
         _check(lib().icempm_partition_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
-        return chosen.value, list(out)
+        return chosen.value, list(out)[:28]
+
+    def tile_config(self):
+        """{tx, ty, margin, lane_spread} of the binned kernels."""
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().icempm_partition_get_tile_config(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("tx", "ty", "margin", "lane_spread"), (x.value for x in v)))
+
+    def graph_stats(self):
+        """(graphs captured since creation, graphs cached now)."""
+        n, c = C.c_longlong(), C.c_int()
+        _check(lib().icempm_partition_get_graph_stats(self._h, C.byref(n), C.byref(c)))
+        return n.value, c.value
 
     def indenter_force(self, n_subdivisions=100):
         out = np.zeros(2 * n_subdivisions)
@@ -371,11 +397,47 @@ class Batch:
         _check(lib().icempm_batch_get_scene_count(self._h, C.byref(n), C.byref(t)))
         return n.value, t.value
 
+    def scene_points(self, scene_id):
+        """Points scene_id holds (after the last compaction)."""
+        n = C.c_int()
+        _check(lib().icempm_batch_get_scene_point_count(self._h, scene_id, C.byref(n)))
+        return n.value
+
+    def get_bins(self, scene_id, n, n_cells):
+        """(sorted keys, previous index of each sorted point, cell counts) of scene_id."""
+        keys = np.zeros(max(n, 1), dtype=np.int32)
+        perm = np.zeros(max(n, 1), dtype=np.int32)
+        counts = np.zeros(max(n_cells, 1), dtype=np.int32)
+        i32 = C.POINTER(C.c_int32)
+        _check(lib().icempm_batch_get_bins(self._h, scene_id, keys.ctypes.data_as(i32), perm.ctypes.data_as(i32), n,
+                                           counts.ctypes.data_as(i32), n_cells))
+        return keys[:n], perm[:n], counts[:n_cells]
+
+    def get_grid(self, scene_id, buf, n_nodes):
+        out = np.zeros((3, n_nodes))
+        _check(lib().icempm_batch_get_grid(self._h, scene_id, buf, _dptr(out), n_nodes))
+        return out
+
+    def position_map(self, scene_id):
+        fb, ox, oy = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().icempm_batch_get_position_map(self._h, scene_id, C.byref(fb), C.byref(ox), C.byref(oy)))
+        return fb.value, ox.value, oy.value
+
     def autotune(self, n_trials=3):
         out = (C.c_float * 32)()
         chosen = C.c_int(-1)
         _check(lib().icempm_batch_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
-        return chosen.value, list(out)
+        return chosen.value, list(out)[:28]
+
+    def tile_config(self):
+        v = [C.c_int() for _ in range(4)]
+        _check(lib().icempm_batch_get_tile_config(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("tx", "ty", "margin", "lane_spread"), (x.value for x in v)))
+
+    def graph_stats(self):
+        n, c = C.c_longlong(), C.c_int()
+        _check(lib().icempm_batch_get_graph_stats(self._h, C.byref(n), C.byref(c)))
+        return n.value, c.value
 
     def kernel_stats(self):
         ms = np.zeros(len(KERNEL_CLASSES))

@@ -358,6 +358,20 @@ int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, 
     return guarded([&] { return p->eng->autotune(n_trials, out_ms, n_out, chosen); });
 }
 
+int icempm_partition_get_tile_config(icempm_partition *p, int *tx, int *ty, int *margin, int *lane_spread)
+{
+    if (!p) return bad("null partition");
+    return p->eng->get_tile_config(tx, ty, margin, lane_spread);
+}
+
+int icempm_partition_get_graph_stats(icempm_partition *p, long long *captures, int *cached)
+{
+    if (!p) return bad("null partition");
+    if (captures) *captures = p->eng->graph_captures();
+    if (cached) *cached = p->eng->cached_graphs();
+    return ICEMPM_OK;
+}
+
 // ---------------------------------------------------------------- batch
 
 int icempm_batch_create(int device, int precision, icempm_batch **out)
@@ -463,6 +477,36 @@ int icempm_batch_get_indenter_force(icempm_batch *b, int scene_id, double *out, 
     return guarded([&] { return b->eng->get_indenter_force(scene_id, out, n_doubles); });
 }
 
+int icempm_batch_get_scene_point_count(icempm_batch *b, int scene_id, int *n_points)
+{
+    if (!b || !n_points) return bad("null argument");
+    if (scene_id < 0 || scene_id >= b->eng->nscenes()) return bad("bad scene id");
+    *n_points = b->eng->scene(scene_id).n_pts;
+    return ICEMPM_OK;
+}
+
+int icempm_batch_get_bins(icempm_batch *b, int scene_id, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts,
+                          int n_cells)
+{
+    if (!b) return bad("null batch");
+    if (scene_id < 0 || scene_id >= b->eng->nscenes()) return bad("bad scene id");
+    return guarded([&] { return b->eng->get_bins(scene_id, keys, perm, n, cell_counts, n_cells); });
+}
+
+int icempm_batch_get_grid(icempm_batch *b, int scene_id, int buf, double *out, long long n_nodes)
+{
+    if (!b || !out) return bad("null argument");
+    if (scene_id < 0 || scene_id >= b->eng->nscenes()) return bad("bad scene id");
+    return guarded([&] { return b->eng->get_grid(scene_id, buf, out, n_nodes); });
+}
+
+int icempm_batch_get_position_map(icempm_batch *b, int scene_id, int *frac_bits, int *origin_x, int *origin_y)
+{
+    if (!b || !frac_bits || !origin_x || !origin_y) return bad("null argument");
+    if (scene_id < 0 || scene_id >= b->eng->nscenes()) return bad("bad scene id");
+    return guarded([&] { return b->eng->get_position_map(scene_id, frac_bits, origin_x, origin_y); });
+}
+
 int icempm_batch_get_error_code(icempm_batch *b, uint8_t *code)
 {
     if (!b || !code) return bad("null argument");
@@ -485,6 +529,20 @@ int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_ou
         return ICEMPM_ERR_STATE;
     }
     return guarded([&] { return b->eng->autotune(n_trials, out_ms, n_out, chosen); });
+}
+
+int icempm_batch_get_tile_config(icempm_batch *b, int *tx, int *ty, int *margin, int *lane_spread)
+{
+    if (!b) return bad("null batch");
+    return b->eng->get_tile_config(tx, ty, margin, lane_spread);
+}
+
+int icempm_batch_get_graph_stats(icempm_batch *b, long long *captures, int *cached)
+{
+    if (!b) return bad("null batch");
+    if (captures) *captures = b->eng->graph_captures();
+    if (cached) *cached = b->eng->cached_graphs();
+    return ICEMPM_OK;
 }
 
 int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep)

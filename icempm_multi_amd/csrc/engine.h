@@ -7,9 +7,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <string>
-#include <tuple>
 #include <vector>
 
 #include "../../include/icempm.h"
@@ -39,6 +39,9 @@ struct SceneHost {
 
 class Engine {
 public:
+    // Units of a substep call (see emit_units in engine.hip).
+    enum UnitKind { kUnitHead = 0, kUnitFused, kUnitTail, kUnitUnfused, kUnitDirect };
+
     Engine(int device, int precision);
     ~Engine();
 
@@ -81,11 +84,16 @@ public:
     int nscenes() const { return (int)scenes_.size(); }
     long long total_points() const;
     float last_step_ms() const { return last_step_ms_; }
+    long long graph_captures() const { return n_captures_; }
+    int cached_graphs() const { return (int)graphs_.size(); }
     void clock(int scene, double *t, int *step) const;
 
     int tx() const { return tx_; }
     int ty() const { return ty_; }
+    int margin() const { return margin_; }
+    int lane_spread() const { return lane_spread_; }
     int set_tile_shape(int tx, int ty, int margin);
+    int get_tile_config(int *tx, int *ty, int *margin, int *lane_spread) const;
     void set_lane_spread(int v) { lane_spread_ = v < 0 ? 0 : (v > 2 ? 2 : v); invalidate_graphs(); }
     void set_dyn_tiles(int v) { dyn_tiles_ = v ? 1 : 0; invalidate_graphs(); }
 
@@ -100,6 +108,14 @@ private:
     StepArgs args() const;
     // timed: record hipEvents around the launches of every sample-th substep
     int launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed, int sample = 1);
+    struct GraphExec;
+    // n units of one kind from grid buffer b and clock slot cs; unit j is timed
+    // when timed and (k0 + j) % sample == 0
+    int emit_units(int kind, int n, int b, int cs, bool timed, int sample, int k0);
+    int chunk_graph(int kind, int n, int b, int cs, int tmode, bool first_timed, GraphExec *&out);
+    int run_units(int kind, int n, int &b, int &cs, bool use_graph, int tmode, int k0,
+                  std::vector<GraphExec *> &replayed);
+    int read_graph_events(const GraphExec &gx);
     int zero_grid(int scene, int buf);
     int record(int cls, bool timed, hipError_t (*)(void *), void *);
     void timed_begin(bool timed);
@@ -160,16 +176,21 @@ private:
     bool capturing_ = false;                 // a graph capture is in progress
     static constexpr int kTimingSample = 32;  // ICEMPM_STEP_SAMPLED: one substep in 32 is timed
 
-    // Graphs keyed by (nsub, start_buf, mode bits).  A timed graph owns the
-    // hipEvents its capture recorded (event record nodes); they are read after
-    // each replay.
+    // Graphs of chunks of 1, 2, 4, ..., kMaxChunk units, keyed by (unit kind,
+    // units, grid buffer, clock slot, point buffer parity, timing mode, first
+    // unit timed).  A timed graph owns the hipEvents its capture recorded
+    // (event record nodes); they are read after each replay.
     struct GraphExec {
         hipGraphExec_t exec = nullptr;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
         std::vector<int> cls;
     };
-    std::map<std::tuple<int, int, int>, GraphExec> graphs_;
-    const GraphExec *graph_read_ = nullptr;   // the graph launched by the current step()
+    typedef std::array<int, 7> GraphKey;
+    static constexpr int kMaxChunk = 64;      // units per graph: at most 128 kernel nodes (3 x 64 unfused)
+    static constexpr size_t kMaxGraphs = 256; // the cache is cleared beyond this
+    std::map<GraphKey, GraphExec> graphs_;
+    int pts_flip_ = 0;                        // which buffer of the pair pts_[0] is (the sort swaps them)
+    long long n_captures_ = 0;                // graphs captured since creation
 };
 
 }  // namespace icempm

@@ -19,6 +19,12 @@
 
 namespace icempm {
 
+// Tile candidates of the autotuner: {tx, ty, lane spread, dynamic queues}.
+const int kTileCandidates[][4] = {{4, 16, 1, 0}, {8, 8, 1, 0},  {4, 32, 1, 0},  {4, 64, 1, 0}, {8, 16, 1, 0},
+                                  {8, 32, 1, 0}, {16, 16, 1, 0}, {4, 32, 0, 0}, {8, 32, 0, 0}, {4, 32, 2, 0},
+                                  {8, 32, 2, 0}, {16, 32, 1, 0}, {8, 64, 1, 0}, {32, 16, 1, 0}};
+const int kNumTileCandidates = (int)(sizeof(kTileCandidates) / sizeof(kTileCandidates[0]));
+
 namespace {
 
 inline long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
@@ -879,18 +885,28 @@ int Engine::build_tiles()
     return ICEMPM_OK;
 }
 
+// Captured chunk graphs survive a sort: the binned kernels read the tile
+// table, cell runs and point counts from device memory, and the swapped point
+// buffer is part of the graph key.  Only the unbinned launches take point
+// counts by value, so a sort that drops disabled points clears the cache.
 int Engine::sort()
 {
     RC(ensure_committed());
-    invalidate_graphs();
     StepArgs a = args();
     timed_begin(false);
     HIPCHK(launch_sort(a, host_dev_.data(), nscenes(), bins_, elem_bytes(), pts_[0], pts_[1], stream_));
     std::swap(pts_[0], pts_[1]);
+    pts_flip_ ^= 1;
     std::vector<int> cnt(nscenes());
     HIPCHK(hipMemcpyAsync(cnt.data(), bins_.scene_count, sizeof(int) * nscenes(), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    for (int i = 0; i < nscenes(); i++) scenes_[i].n_pts = scenes_[i].n_pts > 0 ? cnt[i] : 0;
+    bool counts_changed = false;
+    for (int i = 0; i < nscenes(); i++) {
+        const int m = scenes_[i].n_pts > 0 ? cnt[i] : 0;
+        counts_changed = counts_changed || m != scenes_[i].n_pts;
+        scenes_[i].n_pts = m;
+    }
+    if (counts_changed) invalidate_graphs();
     RC(push_scenes());
     RC(build_tiles());
     bins_valid_ = true;
@@ -920,25 +936,34 @@ int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *ce
 // ---------------------------------------------------------------------------
 // frame stepping
 // ---------------------------------------------------------------------------
-int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed_all, int sample)
+// A call of nsub substeps is a sequence of segments of one kind of unit:
+//   binned, fused:   Head [P2G(b)], nsub-1 x Fused [update(b), G2P(b)+P2G(b^1)],
+//                    Tail [update(b), G2P(b)]
+//   binned, unfused: nsub x Unfused [P2G(b), update(b), G2P(b)]
+//   unbinned:        nsub x Direct [zero, P2G direct, update, G2P direct]
+// Every unit but the head advances the clock slot; the Fused and Unfused units
+// also swap the grid buffers.  The update of a substep reads clock slot cs and
+// writes the advanced clock to cs^1, which the later launches of the substep read.
+int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sample, int k0)
 {
-    // clocks: the update of substep k reads slot cs and writes the advanced
-    // clock to slot cs^1, which the later launches of the substep read
     StepArgs a = args();
     SceneClock *slot[2] = {d_clocks_, d_clocks_ + nscenes()};
-    int cs = clock_slot_;
     const int nb_upd = update_blocks(n_chunks_);
-    int b = start_buf;
-    for (int k = 0; k < nsub; k++) {
-        const bool timed = timed_all && (sample <= 1 || k % sample == 0);
-        StepArgs au = a;
+    for (int j = 0; j < n; j++) {
+        const bool timed = timed_all && (sample <= 1 || (k0 + j) % sample == 0);
+        StepArgs au = a;               // the substep's clock
         au.clocks = slot[cs];
         au.clocks_next = slot[cs ^ 1];
-        StepArgs an = a;               // after the update: the new clock
+        StepArgs an = a;               // after the update: the advanced clock
         an.clocks = slot[cs ^ 1];
         an.clocks_next = slot[cs];
-        cs ^= 1;
-        if (direct) {
+        switch (kind) {
+        case kUnitHead:
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, tile_blocks(kTileP2G), stream_));
+            timed_end(timed, kClsP2G);
+            continue;                  // part of substep 0: no clock or buffer change
+        case kUnitDirect:
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++) {
                 const SceneHost &s = scenes_[i];
@@ -959,17 +984,20 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
             for (int i = 0; i < nscenes(); i++)
                 HIPCHK(LAUNCH_PREC(launch_g2p_direct, an, i, scenes_[i].n_pts, 0, -1, stream_));
             timed_end(timed, kClsG2P);
+            cs ^= 1;
             continue;
-        }
-        if (k == 0 || !fused) {
+        case kUnitUnfused:
             timed_begin(timed);
             HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, tile_blocks(kTileP2G), stream_));
             timed_end(timed, kClsP2G);
+            break;
+        default:
+            break;
         }
         timed_begin(timed);
         HIPCHK(LAUNCH_PREC(launch_update, au, b, 1, 1, nb_upd, stream_));
         timed_end(timed, kClsUpdate);
-        if (fused && k < nsub - 1) {
+        if (kind == kUnitFused) {
             timed_begin(timed);
             HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileFused, b, b ^ 1, -1, tile_blocks(kTileFused), stream_));
             timed_end(timed, kClsFused);
@@ -978,7 +1006,133 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
             HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, -1, tile_blocks(kTileG2P), stream_));
             timed_end(timed, kClsG2P);
         }
+        cs ^= 1;
         b ^= 1;
+    }
+    return ICEMPM_OK;
+}
+
+// Clock slot and grid buffer after n units of a kind (see emit_units).
+static void advance_units(int kind, int n, int &b, int &cs)
+{
+    if (kind == Engine::kUnitHead) return;
+    cs ^= n & 1;
+    if (kind != Engine::kUnitDirect) b ^= n & 1;
+}
+
+// The graph of n units of a kind from (b, cs), captured on first use.  The
+// cache holds chunks of 1, 2, 4, ..., kMaxChunk units, so its size is bounded
+// whatever substep counts the callers use; a captured chunk refers to device
+// buffers that only commit(), set_tile_shape() and upload() replace (they
+// clear the cache), and to the point buffer pair member that the sort swaps
+// (pts_flip_ is part of the key).
+int Engine::chunk_graph(int kind, int n, int b, int cs, int tmode, bool first_timed, GraphExec *&out)
+{
+    const int sample = tmode == 2 ? kTimingSample : 1;
+    const GraphKey key{kind, n, b, cs, pts_flip_, tmode, first_timed ? 1 : 0};
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        if (graphs_.size() >= kMaxGraphs) invalidate_graphs();
+        // The capture records the timing events into the graph: it takes them
+        // from the pool (emptied first) and keeps them.
+        if (tmode) RC(collect_timings());
+        // workgroup counts are runtime queries: make them before the capture
+        tile_blocks(kTileG2P);
+        tile_blocks(kTileP2G);
+        tile_blocks(kTileFused);
+        HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
+        // first_timed with sampling: unit 0 of the chunk is a sampled substep
+        // of its call, and so is every unit a multiple of the sample after it
+        const int rc = emit_units(kind, n, b, cs, tmode != 0, sample, first_timed ? 0 : 1);
+        capturing_ = false;
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream_, &g);
+        GraphExec gx;
+        gx.ev.assign(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
+        gx.cls.assign(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
+        ev_pool_.erase(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
+        ev_cls_.erase(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
+        ev_used_ = 0;
+        auto drop = [&]() {
+            for (auto &e : gx.ev) {
+                hipEventDestroy(e.first);
+                hipEventDestroy(e.second);
+            }
+        };
+        if (rc != ICEMPM_OK) { if (g) hipGraphDestroy(g); drop(); return rc; }
+        if (ec != hipSuccess) { drop(); return hipfail(ec, "hipStreamEndCapture"); }
+        const hipError_t ei = hipGraphInstantiate(&gx.exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) { drop(); return hipfail(ei, "hipGraphInstantiate"); }
+        it = graphs_.emplace(key, gx).first;
+        n_captures_++;
+    }
+    out = &it->second;
+    return ICEMPM_OK;
+}
+
+// Adds the event times of a replayed timed graph to the kernel statistics.
+int Engine::read_graph_events(const GraphExec &gx)
+{
+    for (size_t i = 0; i < gx.ev.size(); i++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, gx.ev[i].first, gx.ev[i].second));
+        cls_ms_[gx.cls[i]] += ms;
+        cls_cnt_[gx.cls[i]]++;
+    }
+    return ICEMPM_OK;
+}
+
+// Runs n units of a kind starting at substep k0 of the call, as chunk graph
+// replays (use_graph) or direct launches; b and cs are advanced.
+int Engine::run_units(int kind, int n, int &b, int &cs, bool use_graph, int tmode, int k0,
+                      std::vector<GraphExec *> &replayed)
+{
+    const int sample = tmode == 2 ? kTimingSample : 1;
+    if (!use_graph) {
+        RC(emit_units(kind, n, b, cs, tmode != 0, sample, k0));
+        advance_units(kind, n, b, cs);
+        return ICEMPM_OK;
+    }
+    int k = k0;
+    while (n > 0) {
+        // largest power of two <= n, at most kMaxChunk: a chunk of 32 or more
+        // units then starts at a multiple of 32 within the segment
+        int len = kMaxChunk;
+        while (len > n) len >>= 1;
+        const bool first_timed = tmode == 2 ? (k % kTimingSample) == 0 : tmode != 0;
+        GraphExec *gx = nullptr;
+        RC(chunk_graph(kind, len, b, cs, tmode, first_timed, gx));
+        if (tmode && std::find(replayed.begin(), replayed.end(), gx) != replayed.end()) {
+            // the graph's events still hold its previous replay in this call: read them first
+            HIPCHK(hipStreamSynchronize(stream_));
+            RC(read_graph_events(*gx));
+            replayed.erase(std::find(replayed.begin(), replayed.end(), gx));
+        }
+        HIPCHK(hipGraphLaunch(gx->exec, stream_));
+        if (tmode) replayed.push_back(gx);
+        advance_units(kind, len, b, cs);
+        n -= len;
+        k += len;
+    }
+    return ICEMPM_OK;
+}
+
+// The units of a call of nsub substeps (see emit_units), without graphs.
+int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed, int sample)
+{
+    std::vector<GraphExec *> none;
+    const int tmode = timed ? (sample > 1 ? 2 : 1) : 0;
+    int b = start_buf, cs = clock_slot_;
+    if (direct) {
+        RC(run_units(kUnitDirect, nsub, b, cs, false, tmode, 0, none));
+    } else if (!fused) {
+        RC(run_units(kUnitUnfused, nsub, b, cs, false, tmode, 0, none));
+    } else {
+        RC(run_units(kUnitHead, 1, b, cs, false, tmode, 0, none));
+        RC(run_units(kUnitFused, nsub - 1, b, cs, false, tmode, 0, none));
+        RC(run_units(kUnitTail, 1, b, cs, false, tmode, nsub - 1, none));
     }
     clock_slot_ = cs;
     return ICEMPM_OK;
@@ -993,7 +1147,7 @@ int Engine::step(int nsub, int flags)
     const bool fused = !(flags & ICEMPM_STEP_UNFUSED);
     const bool timed = (flags & ICEMPM_STEP_TIMINGS) != 0;
     const bool use_graph = (flags & ICEMPM_STEP_GRAPH) != 0;
-    const int sample = (timed && (flags & ICEMPM_STEP_SAMPLED)) ? kTimingSample : 1;
+    const int tmode = timed ? ((flags & ICEMPM_STEP_SAMPLED) ? 2 : 1) : 0;
 
     int start_buf = 0;
     if (!direct) {
@@ -1007,65 +1161,25 @@ int Engine::step(int nsub, int flags)
         }
     }
     HIPCHK(hipEventRecord(ev_a_, stream_));
-    const int slot0 = clock_slot_;
-    if (use_graph) {
-        const int tmode = timed ? (sample > 1 ? 2 : 1) : 0;
-        const auto key = std::make_tuple(nsub, start_buf,
-                                         (direct ? 2 : 0) | (fused ? 1 : 0) | (slot0 << 2) | (tmode << 3));
-        auto it = graphs_.find(key);
-        if (it == graphs_.end()) {
-            // The capture records the timing events into the graph: it takes
-            // them from the pool (emptied first) and keeps them.
-            if (timed) RC(collect_timings());
-            // workgroup counts are runtime queries: make them before the capture
-            tile_blocks(kTileG2P);
-            tile_blocks(kTileP2G);
-            tile_blocks(kTileFused);
-            HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            capturing_ = true;
-            const int rc = launch_substeps(nsub, start_buf, fused, direct, timed, sample);
-            capturing_ = false;
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(stream_, &g);
-            GraphExec gx;
-            gx.ev.assign(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
-            gx.cls.assign(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
-            ev_pool_.erase(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
-            ev_cls_.erase(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
-            ev_used_ = 0;
-            auto drop = [&]() {
-                for (auto &e : gx.ev) {
-                    hipEventDestroy(e.first);
-                    hipEventDestroy(e.second);
-                }
-            };
-            if (rc != ICEMPM_OK) { if (g) hipGraphDestroy(g); drop(); return rc; }
-            if (ec != hipSuccess) { drop(); return hipfail(ec, "hipStreamEndCapture"); }
-            const hipError_t ei = hipGraphInstantiate(&gx.exec, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
-            if (ei != hipSuccess) { drop(); return hipfail(ei, "hipGraphInstantiate"); }
-            it = graphs_.emplace(key, gx).first;
-        }
-        HIPCHK(hipGraphLaunch(it->second.exec, stream_));
-        graph_read_ = &it->second;
+    std::vector<GraphExec *> replayed;
+    int b = start_buf, cs = clock_slot_;
+    if (direct) {
+        RC(run_units(kUnitDirect, nsub, b, cs, use_graph, tmode, 0, replayed));
+    } else if (!fused) {
+        RC(run_units(kUnitUnfused, nsub, b, cs, use_graph, tmode, 0, replayed));
     } else {
-        RC(launch_substeps(nsub, start_buf, fused, direct, timed, sample));
+        RC(run_units(kUnitHead, 1, b, cs, use_graph, tmode, 0, replayed));
+        RC(run_units(kUnitFused, nsub - 1, b, cs, use_graph, tmode, 0, replayed));
+        RC(run_units(kUnitTail, 1, b, cs, use_graph, tmode, nsub - 1, replayed));
     }
-    clock_slot_ = slot0 ^ (nsub & 1);   // a replayed graph advances the slot like a direct launch
+    clock_slot_ = cs;
     HIPCHK(hipEventRecord(ev_b_, stream_));
     HIPCHK(hipEventSynchronize(ev_b_));
     HIPCHK(hipEventElapsedTime(&last_step_ms_, ev_a_, ev_b_));
     last_step_ms_ /= nsub;
-    if (timed && use_graph) {
-        // the replay has completed (ev_b_ above): read its events
-        const GraphExec &gx = *graph_read_;
-        for (size_t i = 0; i < gx.ev.size(); i++) {
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, gx.ev[i].first, gx.ev[i].second));
-            cls_ms_[gx.cls[i]] += ms;
-            cls_cnt_[gx.cls[i]]++;
-        }
-    } else if (timed) {
+    if (timed) {
+        // the replays have completed (ev_b_ above): read their events
+        for (GraphExec *gx : replayed) RC(read_graph_events(*gx));
         RC(collect_timings());
     }
 
@@ -1084,8 +1198,10 @@ int Engine::step(int nsub, int flags)
 }
 
 // ---------------------------------------------------------------------------
-// autotuner: time the binned pipeline for each candidate tile shape on a copy
-// of the current state, keep the fastest, restore the state.
+// autotuner: time the fused launches of the binned pipeline for each candidate
+// tile shape, from a state that has drifted for half a frame since its sort
+// (as at the middle of a frame, where points that left their tile window take
+// the global path), keep the fastest and restore the state.
 // ---------------------------------------------------------------------------
 int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
 {
@@ -1097,76 +1213,120 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     // about twice as long at every shape (measured on the 4M-point sheet).
     // Lane spread 1 won at every shape on the 4M-point sheet, so the other maps
     // are tried at two shapes only; the small tiles are for sparse scenes.
-    static const int cand[][4] = {{4, 16, 1, 0}, {8, 8, 1, 0}, {4, 32, 1, 0}, {4, 64, 1, 0}, {8, 16, 1, 0},
-                                  {8, 32, 1, 0}, {16, 16, 1, 0}, {4, 32, 0, 0}, {8, 32, 0, 0}, {4, 32, 2, 0},
-                                  {8, 32, 2, 0}, {16, 32, 1, 0}, {8, 64, 1, 0}, {32, 16, 1, 0}};
-    const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+    const int ncand = kNumTileCandidates;
     const size_t pts_bytes = (size_t)kNumPtsLines * pitch_ * elem_bytes();
-    // backup: points into the sort buffer, clocks and forces on the host
-    HIPCHK(hipMemcpyAsync(pts_[1], pts_[0], pts_bytes, hipMemcpyDeviceToDevice, stream_));
     std::vector<SceneHost> saved = scenes_;
-    const int tx0 = tx_, ty0 = ty_, ls0 = lane_spread_, dy0 = dyn_tiles_;
+    const int tx0 = tx_, ty0 = ty_, ls0 = lane_spread_, dy0 = dyn_tiles_, margin0 = margin_;
+    // the state at the call, restored at the end (the drift may set error bytes)
+    std::vector<unsigned> err0(nscenes());
+    HIPCHK(hipMemcpyAsync(err0.data(), d_err_, sizeof(unsigned) * nscenes(), hipMemcpyDeviceToHost, stream_));
+    void *orig = nullptr;
+    HIPCHK(hipMalloc(&orig, pts_bytes));
+    HIPCHK(hipMemcpyAsync(orig, pts_[0], pts_bytes, hipMemcpyDeviceToDevice, stream_));
+    // drift: half a frame of substeps with the current shape, no re-sort
+    int drift = 0;
+    for (auto &sc : scenes_) drift = std::max(drift, sc.prms.UpdateEveryNthStep / 2);
+    drift = std::min(drift, 250);
+    int rc = drift > 0 ? step(drift, ICEMPM_STEP_GRAPH) : ICEMPM_OK;
+    // the drifted state, restored before each trial, sits in the sort buffer
+    if (rc == ICEMPM_OK) {
+        const hipError_t e = hipMemcpyAsync(pts_[1], pts_[0], pts_bytes, hipMemcpyDeviceToDevice, stream_);
+        if (e != hipSuccess) rc = hipfail(e, "autotune: state copy");
+    }
+    std::vector<SceneHost> drifted = scenes_;
     float best = 1e30f;
-    int best_i = -1;
+    int best_i = -1, best_margin = margin0;
     n_trials = std::max(1, n_trials);
-    // one trial: n_trials fused substeps at a tile configuration, then points
-    // and grids restored; ms = time per substep, or -1 if the shape does not fit
+    // one trial: a warm-up substep, then n_trials fused units timed alone
+    // (the head P2G and the tail G2P are outside the events); then the drifted
+    // points, clocks and clean grids are restored.  ms = time per fused unit,
+    // or -1 if the shape does not fit
     auto trial = [&](const int *cf, int margin, float &ms) -> int {
         lane_spread_ = cf[2];
         dyn_tiles_ = cf[3];
         ms = -1;
         if (set_tile_shape(cf[0], cf[1], margin) != ICEMPM_OK) return ICEMPM_OK;
         RC(step(1, 0));   // warm-up
+        std::vector<GraphExec *> none;
+        int b = scenes_[0].clean[0] ? 0 : 1, cs = clock_slot_;
+        RC(run_units(kUnitHead, 1, b, cs, false, 0, 0, none));
         HIPCHK(hipEventRecord(ev_a_, stream_));
-        RC(launch_substeps(n_trials, scenes_[0].clean[0] ? 0 : 1, true, false, false));
+        RC(run_units(kUnitFused, n_trials, b, cs, false, 0, 0, none));
         HIPCHK(hipEventRecord(ev_b_, stream_));
+        RC(run_units(kUnitTail, 1, b, cs, false, 0, n_trials + 1, none));
+        clock_slot_ = cs;
         HIPCHK(hipEventSynchronize(ev_b_));
         HIPCHK(hipEventElapsedTime(&ms, ev_a_, ev_b_));
         ms /= n_trials;
         HIPCHK(hipMemcpyAsync(pts_[0], pts_[1], pts_bytes, hipMemcpyDeviceToDevice, stream_));
-        for (int i = 0; i < nscenes(); i++) { RC(zero_grid(i, 0)); RC(zero_grid(i, 1)); }
-        return ICEMPM_OK;
+        for (int i = 0; i < nscenes(); i++) {
+            RC(zero_grid(i, 0));
+            RC(zero_grid(i, 1));
+            scenes_[i].sim_time = drifted[i].sim_time;
+            scenes_[i].sim_step = drifted[i].sim_step;
+        }
+        return push_clocks();
     };
-    // Every shape is timed with the current window drift margin and with the
-    // other one (1 or 0).  No margin pays where the points stay in their cells
-    // between sorts (the 4M-point sheet, mixed form: fused launch 103.6 ->
-    // 98.7 us); the margin where they drift (batches of moving blocks, 7%
-    // faster with it).  The best shape differs between the two margins, so
+    // Every shape is timed with a window drift margin of 0 and of 1 (results
+    // in that order).  No margin pays where the points stay in their cells
+    // between sorts (smaller windows: less staging and flushing); the margin
+    // where they drift.  The best shape differs between the two margins, so
     // both are swept in full.
-    const int margin0 = margin_;
-    const int margins[2] = {margin0, margin0 == 0 ? 1 : 0};
-    int best_margin = margin0;
-    for (int mi = 0; mi < 2; mi++) {
-        for (int c = 0; c < ncand; c++) {
+    for (int mg = 0; mg < 2 && rc == ICEMPM_OK; mg++) {
+        for (int c = 0; c < ncand && rc == ICEMPM_OK; c++) {
             float ms;
-            RC(trial(cand[c], margins[mi], ms));
-            const int k = mi * ncand + c;
+            rc = trial(kTileCandidates[c], mg, ms);
+            const int k = mg * ncand + c;
             if (k < n_out && out_ms) out_ms[k] = ms;
-            if (ms >= 0 && ms < best) {
+            if (rc == ICEMPM_OK && ms >= 0 && ms < best) {
                 best = ms;
                 best_i = c;
-                best_margin = margins[mi];
+                best_margin = mg;
             }
         }
+    }
+    // the state at the call
+    {
+        const hipError_t e = hipMemcpyAsync(pts_[0], orig, pts_bytes, hipMemcpyDeviceToDevice, stream_);
+        const hipError_t e2 =
+            hipMemcpyAsync(d_err_, err0.data(), sizeof(unsigned) * nscenes(), hipMemcpyHostToDevice, stream_);
+        hipStreamSynchronize(stream_);
+        hipFree(orig);
+        if (rc == ICEMPM_OK && e != hipSuccess) rc = hipfail(e, "autotune: state restore");
+        if (rc == ICEMPM_OK && e2 != hipSuccess) rc = hipfail(e2, "autotune: error byte restore");
     }
     for (int i = 0; i < nscenes(); i++) {
         scenes_[i].sim_time = saved[i].sim_time;
         scenes_[i].sim_step = saved[i].sim_step;
+        if (rc == ICEMPM_OK) {
+            rc = zero_grid(i, 0);
+            if (rc == ICEMPM_OK) rc = zero_grid(i, 1);
+        }
     }
-    RC(push_clocks());
-    RC(reset_indenter_force());
-    if (best_i < 0) {
+    if (best_i < 0 || rc != ICEMPM_OK) {
         lane_spread_ = ls0;
         dyn_tiles_ = dy0;
         RC(set_tile_shape(tx0, ty0, margin0));
     } else {
-        lane_spread_ = cand[best_i][2];
-        dyn_tiles_ = cand[best_i][3];
-        RC(set_tile_shape(cand[best_i][0], cand[best_i][1], best_margin));
+        lane_spread_ = kTileCandidates[best_i][2];
+        dyn_tiles_ = kTileCandidates[best_i][3];
+        RC(set_tile_shape(kTileCandidates[best_i][0], kTileCandidates[best_i][1], best_margin));
     }
     invalidate_graphs();   // captured launches hold the lane map and distribution by value
-    if (chosen) *chosen = best_i < 0 ? -1 : best_i + (best_margin == margin0 ? 0 : ncand);
+    RC(rc);
+    RC(push_clocks());
+    RC(reset_indenter_force());
+    if (chosen) *chosen = best_i < 0 ? -1 : best_margin * ncand + best_i;
     HIPCHK(hipStreamSynchronize(stream_));
+    return ICEMPM_OK;
+}
+
+int Engine::get_tile_config(int *tx, int *ty, int *margin, int *lane_spread) const
+{
+    if (tx) *tx = tx_;
+    if (ty) *ty = ty_;
+    if (margin) *margin = margin_;
+    if (lane_spread) *lane_spread = lane_spread_;
     return ICEMPM_OK;
 }
 

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "model.h"
 
@@ -150,4 +151,27 @@ void GPU_Implementation5::step(int n_substeps, int flags)
 void GPU_Implementation5::sort_points()
 {
     for (GPU_Partition &p : partitions) p.sort_points();
+}
+
+int GPU_Implementation5::rebalance_on_device()
+{
+    if (partitions.size() != 1) throw std::runtime_error("rebalance_on_device: one partition only");
+    GPU_Partition &p = partitions[0];
+    int n0 = 0, nd0 = 0;
+    if (icempm_partition_get_point_count(p.handle, &n0, &nd0) != ICEMPM_OK)
+        throw std::runtime_error(icempm_last_error());
+    if ((unsigned)n0 != hssoa.size)
+        throw std::runtime_error("rebalance_on_device: the host copy does not hold the device's points");
+    p.sort_points();
+    int n = 0, nd = 0;
+    if (icempm_partition_get_point_count(p.handle, &n, &nd) != ICEMPM_OK)
+        throw std::runtime_error(icempm_last_error());
+    std::vector<int32_t> perm((size_t)std::max(n, 1));
+    if (icempm_partition_get_bins(p.handle, nullptr, perm.data(), n, nullptr, 0) != ICEMPM_OK)
+        throw std::runtime_error(icempm_last_error());
+    hssoa.ApplyPermutation(perm.data(), (unsigned)n);
+    p.nPts_partition = n;
+    p.nPts_disabled = 0;
+    device_rebalances++;
+    return n;
 }

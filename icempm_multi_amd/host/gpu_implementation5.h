@@ -45,6 +45,13 @@ public:
     // recordPQ cadence and clock kept on the device (no reference counterpart).
     void step(int n_substeps, int flags);
     void sort_points();                       // device RemoveDisabledAndSort
+    // Rebalance of one partition on the device (replaces the host
+    // RemoveDisabledAndSort + split + re-upload of model.cpp:102-110): the
+    // device drops disabled points and re-sorts, and the host copy, which holds
+    // the frame just downloaded, is permuted the same way (4 bytes per point
+    // read back, nothing uploaded).  Returns the points kept.
+    int rebalance_on_device();
+    int device_rebalances = 0;                // rebalances done by rebalance_on_device
 
     int precision = ICEMPM_PRECISION_FP64;    // precision form of new partitions
 };

@@ -23,6 +23,7 @@ GPU_Partition &GPU_Partition::operator=(GPU_Partition &&o) noexcept
     handle = o.handle;
     o.handle = nullptr;
     PartitionID = o.PartitionID;
+    uploads = o.uploads;
     Device = o.Device;
     Precision = o.Precision;
     nPtsPitch = o.nPtsPitch;
@@ -82,6 +83,7 @@ void GPU_Partition::transfer_points_from_soa_to_device(HostSideSOA &hssoa, int p
                                                               point_idx_offset, nPts_partition),
           "transfer_points_from_soa_to_device");
     nPts_disabled = 0;
+    uploads++;
 }
 
 void GPU_Partition::transfer_from_device(HostSideSOA &hssoa, int point_idx_offset)

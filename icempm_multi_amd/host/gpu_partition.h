@@ -68,6 +68,7 @@ public:
           timing_60_ptsSent = 0, timing_70_ptsAccepted = 0, timing_stepTotal = 0;
 
     icempm_partition *handle = nullptr;
+    int uploads = 0;                       // transfer_points_from_soa_to_device calls (no reference counterpart)
 
 private:
     void check(int rc, const char *what) const;

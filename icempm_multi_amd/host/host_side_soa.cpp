@@ -67,6 +67,38 @@ void HostSideSOA::RemoveDisabledAndSort(double hinv, unsigned GridY)
     size = (unsigned)idx.size();
 }
 
+void HostSideSOA::ApplyPermutation(const int32_t *perm, unsigned n)
+{
+    if (n > size) throw std::runtime_error("ApplyPermutation: more points than the buffer holds");
+    for (unsigned k = 0; k < n; k++)
+        if (perm[k] < 0 || (unsigned)perm[k] >= size) throw std::runtime_error("ApplyPermutation: index out of range");
+    std::vector<double> tmp(n);
+    for (size_t line = 0; line < icy::SimParams::nPtsArrays; line++) {
+        double *L = getPointerToLine((int)line);
+        for (unsigned k = 0; k < n; k++) tmp[k] = L[perm[k]];
+        std::copy(tmp.begin(), tmp.end(), L);
+    }
+    size = n;
+}
+
+unsigned HostSideSOA::FindFirstPointAtGridXIndex(const int index_grid_x, const double hinv)
+{
+    const double *X = getPointerToPosX();
+    unsigned lo = 0, hi = size;
+    while (lo < hi) {
+        const unsigned mid = lo + (hi - lo) / 2;
+        if ((int)(X[mid] * hinv + 0.5) < index_grid_x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+std::pair<Vec2, Vec2> HostSideSOA::getBlockDimensions()
+{
+    const auto lo = getBlockMin(), hi = getBlockMax();
+    return {Vec2(lo.first, lo.second), Vec2(hi.first, hi.second)};
+}
+
 void HostSideSOA::InitializeBlock()
 {
     for (unsigned i = 0; i < size; i++) {

@@ -57,7 +57,7 @@ int usage()
 {
     std::fprintf(stderr, "usage: icempm_run CONFIG.json [--points FILE.h5] [--frames N] [--precision fp64|mixed]\n"
                          "       [--host-loop] [--no-resort] [--kernel-timings] [--log-dir DIR] [--snapshot-dir DIR]\n"
-                         "       [--resume SNAPSHOT.h5] [--dump-soa FILE] [--quiet]\n"
+                         "       [--resume SNAPSHOT.h5] [--dump-soa FILE] [--quiet] [--disable-every K]\n"
                          "       icempm_run --convert-raw IN.bin OUT.h5 | --read-raw IN.h5 OUT.bin\n");
     return 2;
 }
@@ -108,7 +108,7 @@ int main(int argc, char **argv)
         }
 
         std::string config = a1, points, log_dir = "logs", snapshot_dir, resume, dump;
-        int frames = -1;
+        int frames = -1, disable_every = 0;
         icy::Model model;
         icy::SnapshotManager snapshot;
         snapshot.model = &model;
@@ -134,6 +134,7 @@ int main(int argc, char **argv)
             else if (o == "--resume") resume = next();
             else if (o == "--dump-soa") dump = next();
             else if (o == "--quiet") model.verbose = false;
+            else if (o == "--disable-every") disable_every = std::atoi(next().c_str());
             else return usage();
         }
         model.logDirectory = log_dir;
@@ -141,6 +142,18 @@ int main(int argc, char **argv)
         const std::string raw = model.prms.ParseFile(config);
         if (!resume.empty()) snapshot.ReadSnapshot(resume);
         else snapshot.LoadRawPoints(points.empty() ? raw : points);
+        if (disable_every > 0) {
+            // test hook: every k-th loaded point disabled (as a point that left
+            // the domain would be), the active count adjusted, points re-uploaded
+            HostSideSOA &S = model.gpu.hssoa;
+            int active = 0;
+            for (unsigned i = 0; i < S.size; i++) {
+                if (i % (unsigned)disable_every == 0) S.setUtility(i, S.utility(i) | ICEMPM_STATUS_DISABLED);
+                else active++;
+            }
+            model.prms.nPtsTotal = active;
+            model.gpu.transfer_ponts_to_device();
+        }
 
         bool more = true;
         for (int f = 0; more && (frames < 0 || f < frames); f++) more = model.Step();
@@ -155,8 +168,10 @@ int main(int argc, char **argv)
             o.write((const char *)S.host_buffer, sizeof(double) * (size_t)S.capacity * icy::SimParams::nPtsArrays);
             if (!o) throw std::runtime_error("writing " + dump);
         }
-        std::printf("done: step %d time %.6f points %u\n", model.prms.SimulationStep, model.prms.SimulationTime,
-                    model.gpu.hssoa.size);
+        int uploads = 0;
+        for (const GPU_Partition &p : model.gpu.partitions) uploads += p.uploads;
+        std::printf("done: step %d time %.6f points %u uploads %d device_rebalances %d\n", model.prms.SimulationStep,
+                    model.prms.SimulationTime, model.gpu.hssoa.size, uploads, model.gpu.device_rebalances);
         return 0;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "icempm_run: %s\n", e.what());

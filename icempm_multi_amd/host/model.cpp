@@ -121,10 +121,20 @@ bool icy::Model::Step()
     }
 
     if (rebalance) {
-        if (verbose) std::printf("squeezing and sorting HSSOA\n");
-        gpu.hssoa.RemoveDisabledAndSort(prms.cellsize_inv, prms.GridY);
-        gpu.split_hssoa_into_partitions();
-        gpu.transfer_ponts_to_device();
+        if (gpu.partitions.size() == 1) {
+            // one partition: compaction and sort on the device, the host copy
+            // permuted to match (no re-upload of the 16 point lines)
+            const unsigned before = gpu.hssoa.size;
+            const int kept = gpu.rebalance_on_device();
+            if (verbose)
+                std::printf("rebalancing on device: %d points kept, %u disabled dropped, no re-upload\n", kept,
+                            before - (unsigned)kept);
+        } else {
+            if (verbose) std::printf("squeezing and sorting HSSOA\n");
+            gpu.hssoa.RemoveDisabledAndSort(prms.cellsize_inv, prms.GridY);
+            gpu.split_hssoa_into_partitions();
+            gpu.transfer_ponts_to_device();
+        }
         SyncTopologyRequired = true;
         if (verbose) std::printf("rebalancing done\n");
     }

@@ -118,7 +118,7 @@ typedef struct icempm_simparams {
 #define ICEMPM_DEVERR_NO_SPACE  5
 
 /* Flags for the frame-level step calls */
-#define ICEMPM_STEP_GRAPH       0x1   /* capture the substeps into a hipGraph and replay it */
+#define ICEMPM_STEP_GRAPH       0x1   /* run the substeps as replays of hipGraphs of 1-64 substeps */
 #define ICEMPM_STEP_RESORT      0x2   /* re-bin and re-sort points on device before the substeps */
 #define ICEMPM_STEP_TIMINGS     0x4   /* record per-phase hipEvents (timing_* fields) */
 #define ICEMPM_STEP_DIRECT_P2G  0x8   /* force the unbinned one-thread-per-point P2G */
@@ -258,12 +258,22 @@ int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread);
  * tile list, 1 = per-XCD atomic tile queues with stealing.  Set by the autotuner. */
 int icempm_partition_set_tile_schedule(icempm_partition *p, int dynamic);
 
-/* Project-written autotuner: times the binned P2G / fused G2P launch shapes on
- * the current point set, each with the current window drift margin and with
- * the other one (1 or 0), and keeps the fastest.  Results (ms per launch; -1
- * for a shape that does not fit) go to out_ms if non-null: the candidates at
- * the current margin, then at the other.  *chosen = the index kept. */
+/* Project-written autotuner: times the fused G2P+P2G launch of every candidate
+ * tile shape, each with a window drift margin of 0 and of 1, on the current
+ * point set after half a frame of substeps (the state drifts from its sort as
+ * it does mid-frame), keeps the fastest and restores the state.  Results (ms
+ * per fused launch; -1 for a shape that does not fit) go to out_ms if
+ * non-null: the candidates at margin 0, then at margin 1 (28 values).
+ * *chosen = margin * 14 + candidate, the shape kept; the shape itself is read
+ * with icempm_partition_get_tile_config. */
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen);
+/* The tile shape, window drift margin and lane map in use (no reference counterpart). */
+int icempm_partition_get_tile_config(icempm_partition *p, int *tx, int *ty, int *margin, int *lane_spread);
+/* hipGraph cache of the stepping calls (no reference counterpart): graphs
+ * captured since creation and graphs held now.  A call of n substeps with
+ * ICEMPM_STEP_GRAPH replays graphs of 1, 2, 4, ..., 64 substeps, captured on
+ * first use and kept across calls and re-sorts. */
+int icempm_partition_get_graph_stats(icempm_partition *p, long long *captures, int *cached);
 
 /* ------------------------------------------------------------------------- */
 /* Batch: many independent scenes stepped by one persistent launch per phase   */
@@ -284,9 +294,20 @@ int icempm_batch_step(icempm_batch *b, int n_substeps, int flags);
 int icempm_batch_transfer_from_device(icempm_batch *b, int scene_id, double *hssoa, unsigned hssoa_capacity);
 int icempm_batch_get_indenter_force(icempm_batch *b, int scene_id, double *out, int n_doubles);
 int icempm_batch_get_error_code(icempm_batch *b, uint8_t *code);
+/* Per-scene inspection, as the partition functions of the same names (no
+ * reference counterpart): points held (after the last compaction), the bins of
+ * the last sort, a grid buffer, the mixed form's position map. */
+int icempm_batch_get_scene_point_count(icempm_batch *b, int scene_id, int *n_points);
+int icempm_batch_get_bins(icempm_batch *b, int scene_id, int32_t *keys, int32_t *perm, int n, int32_t *cell_counts,
+                          int n_cells);
+int icempm_batch_get_grid(icempm_batch *b, int scene_id, int buf, double *out, long long n_nodes);
+int icempm_batch_get_position_map(icempm_batch *b, int scene_id, int *frac_bits, int *origin_x, int *origin_y);
 int icempm_batch_get_scene_count(icempm_batch *b, int *n_scenes, long long *n_points_total);
 /* The autotuner of icempm_partition_autotune over all scenes of the batch. */
 int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_out, int *chosen);
+/* As the partition functions of the same names. */
+int icempm_batch_get_tile_config(icempm_batch *b, int *tx, int *ty, int *margin, int *lane_spread);
+int icempm_batch_get_graph_stats(icempm_batch *b, long long *captures, int *cached);
 /* Average device time per substep (ms) of the last icempm_batch_step call. */
 int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep);
 /* Device time per kernel class of the batch (classes as for partitions),

@@ -139,3 +139,47 @@ def test_negative_lengths_are_argument_errors():
     part.step(3, 0)
     assert part.error_code == 0
     part.close()
+
+
+def test_sort_keys_of_points_beyond_the_grid():
+    """The device key of a point beyond the grid is the key of the nearest
+    cell the scene's cell table holds (columns -halo .. GridX+halo-1, rows
+    0 .. GridY-1): the oracle key of the point projected there.  The
+    reference's key (proxypoint2d.cpp:164-169) is not bounded and would
+    index outside the table; such points also set the error byte when
+    stepped.  Points inside the grid keep the oracle's key and order.  Only
+    the sort runs here."""
+    P, h, n = moving_block(0x1CE0A20)
+    hh, gy, gx = P.GridHaloSize, P.GridY, P.GridXTotal
+    cs = P.cellsize
+    outside = [(-30 * cs, 5.3 * cs), ((gx + 30) * cs, 7.6 * cs), (20.2 * cs, -5 * cs), (31.7 * cs, (gy + 7) * cs),
+               (-50 * cs, -50 * cs), ((gx + 50) * cs, (gy + 50) * cs), (-(hh + 3) * cs, 10 * cs)]
+    m = n + len(outside)
+    hs = np.zeros((16, m + 8))
+    hs[:, :n] = h[:, :n]
+    for k, (x, y) in enumerate(outside):
+        hs[:, n + k] = h[:, k]
+        hs[4, n + k], hs[5, n + k] = x, y
+    P.nPtsTotal = m
+    part = make(P, hs, m, im.PRECISION_FP64)
+    part.sort_points()
+    keys, perm, _ = part.get_bins(m, O.n_nodes(P))
+    assert np.array_equal(np.sort(perm), np.arange(m)) and np.all(np.diff(keys) >= 0)
+    # projection onto the table: column 0 is ix = -halo, reached by (int)(x/h + 0.5)
+    # truncating toward zero at x = -(halo + 1) h; the last column at (GridX + halo - 1) h
+    x_lo, x_hi = -(hh + 1) * cs, (gx + hh - 1) * cs
+    y_lo, y_hi = 0.0, (gy - 1) * cs
+    for k in range(m):
+        x, y = hs[4, perm[k]], hs[5, perm[k]]
+        if perm[k] < n:
+            assert keys[k] == O.cell_key(P, x, y)
+        else:
+            xc = x if (x / cs + 0.5 >= -hh and x / cs + 0.5 < gx + hh) else min(max(x, x_lo), x_hi)
+            yc = y if (y / cs + 0.5 >= 0 and y / cs + 0.5 < gy) else min(max(y, y_lo), y_hi)
+            assert keys[k] == O.cell_key(P, xc, yc), (k, x, y, keys[k])
+    # in-grid points: the oracle's stable order
+    ref = hs.copy()
+    _, rkeys, rperm = O.sort(P, ref, n)
+    inside = perm[perm < n]
+    assert np.array_equal(inside, rperm)
+    part.close()

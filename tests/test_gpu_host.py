@@ -85,8 +85,9 @@ def test_model_frames_fp64_match_oracle(tmp_path, mode):
 def test_model_rebalance_path_matches_oracle(tmp_path):
     """ExtraSpaceForIncomingPoints = 0.1 leaves less free point space than
     RebalanceThresholdFreeSpaceRemaining (0.15), so every frame ends with the
-    rebalance of model.cpp:102-110: host RemoveDisabledAndSort, re-split and
-    re-upload.  The run must still follow the oracle."""
+    rebalance of model.cpp:102-110.  With one partition it runs on the device
+    (compaction and sort, the host copy permuted to match): the points are
+    uploaded once, at load, and the run follows the oracle."""
     d = np.load(os.path.join(H.GOLDEN, "ref_2d_70k.npz"))
     H.write_flat_raw(tmp_path / "in.bin", d["x"], d["y"], d["llGrainIDs"], d["volume"][0])
     H.run("--convert-raw", tmp_path / "in.bin", tmp_path / "pts.h5")
@@ -96,12 +97,43 @@ def test_model_rebalance_path_matches_oracle(tmp_path):
     out_s = H.run(cfg, "--points", tmp_path / "pts.h5", "--frames", 2, "--log-dir", "",
                   "--dump-soa", tmp_path / "out.bin", cwd=tmp_path)
     assert out_s.count("rebalancing done") == 2
+    assert out_s.count("rebalancing on device") == 2
+    assert "uploads 1 device_rebalances 2" in out_s
     out, size = H.read_soa_dump(tmp_path / "out.bin")
     assert size == n
     ref, _ = oracle_frames(P, h, n, 2)
-    # the host sort after frame 1 orders the points as the oracle's sort does
+    # the device sort after frame 2 orders the points as the oracle's sort does
     O.sort(P, ref, n)
     assert_close(out, ref, n, 1e-8, "rebalance")
+
+
+def test_device_rebalance_drops_disabled_points(tmp_path):
+    """Points disabled in the input are dropped by the device rebalance, and
+    the host copy keeps exactly the device's points in the device's order: the
+    run equals the oracle on the input with the disabled points removed."""
+    d = np.load(os.path.join(H.GOLDEN, "ref_2d_70k.npz"))
+    H.write_flat_raw(tmp_path / "in.bin", d["x"], d["y"], d["llGrainIDs"], d["volume"][0])
+    H.run("--convert-raw", tmp_path / "in.bin", tmp_path / "pts.h5")
+    cfg = H.p1_config(str(tmp_path), ExtraSpaceForIncomingPoints=0.1)
+    P, _ = im.params_parse_json(open(cfg).read())
+    h, n = scenes.load_raw_points(P, d["x"], d["y"], d["llGrainIDs"], float(d["volume"][0]))
+    # --no-resort: the frame runs on the loaded order with the disabled points
+    # in place, and the rebalance at its end is the first device sort
+    out_s = H.run(cfg, "--points", tmp_path / "pts.h5", "--frames", 1, "--log-dir", "", "--no-resort",
+                  "--dump-soa", tmp_path / "out.bin", "--disable-every", 97, cwd=tmp_path)
+    dropped = len(range(0, n, 97))
+    assert f"{n - dropped} points kept, {dropped} disabled dropped" in out_s
+    assert "uploads 2 device_rebalances 1" in out_s          # load, re-upload with the flags; none after
+    out, size = H.read_soa_dump(tmp_path / "out.bin")
+    assert size == n - dropped
+    ref = h.copy()
+    O.sort(P, ref, n)                                      # the order LoadRawPoints leaves
+    ud = ref[0, :n].view(np.int64)
+    ud[::97] |= 0x20000
+    ref, _ = oracle_frames(P, ref, n, 1, resort=False)
+    m, _, _ = O.sort(P, ref, n)
+    assert m == n - dropped
+    assert_close(out, ref, m, 1e-8, "rebalance with disabled points")
 
 
 def test_model_frames_mixed_accuracy(tmp_path):

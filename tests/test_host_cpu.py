@@ -69,3 +69,18 @@ def test_driver_rejects_bad_arguments(tmp_path):
     assert r.returncode == 2
     r = subprocess.run([H.RUN, str(tmp_path / "missing.json")], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 1 and "configuration file" in r.stderr
+
+
+def test_soa_proxy_iterator(tmp_path):
+    """ProxyPoint / SOAIterator (host_side_soa.h:18-74 of the reference): the
+    reference's remove_if + sort through the iterator moves whole points and
+    agrees with RemoveDisabledAndSort; FindFirstPointAtGridXIndex equals
+    std::lower_bound and a linear scan."""
+    exe = tmp_path / "soa_iterator_test"
+    host = os.path.join(H.ROOT, "icempm_multi_amd", "host")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I" + host, os.path.join(H.ROOT, "tests", "cpp",
+                    "soa_iterator_test.cpp"), "-L" + LIBDIR, "-licempm_host", "-Wl,-rpath," + LIBDIR,
+                    "-Wl,-rpath,/opt/rocm/lib", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok: 0 failures" in r.stdout
