@@ -211,6 +211,24 @@ int Engine::build_scene_dev(int i, SceneDev &d) const
     c.ind_vel = P.IndVelocity;
     c.ind_x0 = P.indenter_x_initial;
     c.ind_y = P.indenter_y;
+    // scene invariants of the point loop, with the operations the reference
+    // applies per point (kernels.cuh:30-33, 552-555, 560-576, 598-619, 636-638)
+    {
+        const double pmax = c.ics, pmin = -c.its;
+        c.s4 = 4. * c.hinv;
+        c.kappa_half = c.kappa * 0.5;
+        c.coeff1_mu = 1.4142135623730950 * c.mu;
+        c.gv033 = c.grain_var * 0.033;
+        c.gv01 = c.grain_var * 0.1;
+        c.b2 = 1 + 2 * c.beta;
+        c.bpmax = c.beta * pmax;
+        c.dp = pmax - pmin;
+        c.dp2 = (pmax - pmin) * (pmax - pmin);
+        c.q0a = 2 * std::sqrt(-pmax * pmin);
+        const double p_new = -c.dp_threshold_p;
+        c.je_tension = std::sqrt(-2. * p_new / c.kappa + 1.);
+        c.sq_je_tension = std::sqrt(c.je_tension);
+    }
     ConstsF &f = c.f;
     f.h = (float)c.h;
     f.hinv = (float)c.hinv;

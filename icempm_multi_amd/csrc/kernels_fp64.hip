@@ -28,17 +28,19 @@ namespace fp64 {
 
 // ---------------------------------------------------------------------------
 // Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
-// 2 waves per SIMD (up to 256 VGPRs; the fused kernel needs about 242): at 3
-// waves (168 VGPRs) it spills 36-64 VGPRs in the point loop, and a spill
-// reload waits for every outstanding load, the prefetch of the next point
-// included (measured slower on the 4M-point sheet).
+// 3 waves per SIMD (up to 168 VGPRs; the fused kernel fits with no spills).
+// What keeps it there: the scene invariants of the point loop (4/h, kappa/2,
+// sqrt(2) mu, the failure-surface and tension-branch terms) come from the host
+// in scalar registers, and the water stress uses J^-5 = 1/(J^2 J^2 J) rather
+// than pow(), whose polynomial constants the compiler would otherwise keep in
+// vector registers across the loop (242 VGPRs, 2 waves per SIMD, before).
 // ---------------------------------------------------------------------------
 //
 // WHC is the window height (ty + 2 + 2*margin) when fixed at compile time for
 // the common tile shapes (0: read from the arguments).  A compile-time height
 // turns the stencil's LDS offsets into instruction immediates.
 template <int MODE, int WHC>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) k_tiles(StepArgs a, int rbuf, int wbuf,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_tiles(StepArgs a, int rbuf, int wbuf,
                                                                                         int record_mode)
 {
     constexpr bool DO_G2P = (MODE & kTileG2P) != 0;

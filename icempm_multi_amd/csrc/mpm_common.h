@@ -56,6 +56,16 @@ struct SceneConsts {
     double ics, its, iss, its2;       // Ice{Compressive,Tensile,Shear,Tensile2}Strength
     double beta, grain_var, tan_phi, dp_threshold_p;
     double ind_rsq, ind_vel, ind_x0, ind_y;
+    // derived on the host with the kernels' own operations (bit-identical
+    // results); read from scalar registers instead of recomputed per tile
+    double s4;                        // 4 * hinv
+    double kappa_half;                // kappa * 0.5
+    double coeff1_mu;                 // sqrt(2) * mu (kernels.cuh:10, 555)
+    double gv033, gv01;               // GrainVariability * 0.033, * 0.1
+    double b2, bpmax;                 // 1 + 2 beta, beta * pmax
+    double dp, dp2;                   // pmax - pmin, (pmax - pmin)^2
+    double q0a;                       // 2 sqrt(-pmax pmin)
+    double je_tension, sq_je_tension; // sqrt(-2 p_new / kappa + 1), its square root (p_new = -DP_threshold_p)
     ConstsF f;                        // mixed form
 };
 

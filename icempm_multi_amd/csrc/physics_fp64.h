@@ -73,17 +73,20 @@ __device__ __forceinline__ void kirchhoff(const SceneConsts &c, const double F[4
     double d[4];
     dev_ffT(F, d);
     const double a = c.mu * (1 / Je);
-    const double k = c.kappa * 0.5 * (Je * Je - 1.);
+    const double k = c.kappa_half * (Je * Je - 1.);
     tau[0] = a * d[0] + k;
     tau[1] = a * d[1];
     tau[2] = a * d[2];
     tau[3] = a * d[3] + k;
 }
 
-// water (kernels.cuh:37-44)
+// water (kernels.cuh:37-44): kappa (1 - J^-5) I, with J^-5 formed as
+// 1/(J^2 J^2 J) (within a few ulps of pow(J, -5.0), whose exp/log polynomial
+// constants would occupy vector registers in the tile kernels)
 __device__ __forceinline__ void water(const SceneConsts &c, double J, double tau[4])
 {
-    const double k = c.kappa * (1. - pow(J, -5.0));
+    const double j2 = J * J;
+    const double k = c.kappa * (1. - 1. / (j2 * j2 * J));
     tau[0] = k; tau[1] = 0; tau[2] = 0; tau[3] = k;
 }
 
@@ -216,12 +219,12 @@ __device__ __forceinline__ void compute_pq(const SceneConsts &c, const double F[
                                            double &q, StressTerms &st)
 {
     Je = det2(F);
-    p = -(c.kappa / 2.) * (Je * Je - 1.);
+    p = -c.kappa_half * (Je * Je - 1.);
     dev_ffT(F, st.d);
     const double *d = st.d;
     const double nrm = sqrt(d[0] * d[0] + d[2] * d[2] + d[1] * d[1] + d[3] * d[3]);
     st.inv_je = 1. / Je;
-    q = kCoeff1 * c.mu * st.inv_je * nrm;
+    q = c.coeff1_mu * st.inv_je * nrm;
     st.p = p;
     st.valid = true;
 }
@@ -236,21 +239,21 @@ __device__ __forceinline__ void compute_pq(const SceneConsts &c, const double F[
 // CheckIfPointIsInsideFailureSurface + GetParametersForGrain (kernels.cuh:560-619)
 __device__ __forceinline__ bool failure(const SceneConsts &c, int grain, double p, double q)
 {
-    const double var2 = 1.0 + c.grain_var * 0.033 * (-15 + (grain + 3) % 30);
-    const double var3 = 1.0 + c.grain_var * 0.1 * (-10 + (grain + 4) % 11);
-    const double pmax = c.ics, pmin = -c.its;
+    // the grain-independent terms come from the host (same operations)
+    const double var2 = 1.0 + c.gv033 * (-15 + (grain + 3) % 30);
+    const double var3 = 1.0 + c.gv01 * (-10 + (grain + 4) % 11);
+    const double pmax = c.ics;
     const double qmax = c.iss * var3;
     const double pmin2 = -c.its2 * var2;
-    const double beta = c.beta;
-    const double mSq = (4 * qmax * qmax * (1 + 2 * beta)) / ((pmax - pmin) * (pmax - pmin));
+    const double mSq = (4 * qmax * qmax * c.b2) / c.dp2;
     if (p < 0) {
         if (p < pmin2) return true;
-        const double q0 = 2 * sqrt(-pmax * pmin) * qmax / (pmax - pmin);
+        const double q0 = c.q0a * qmax / c.dp;
         const double k = -q0 / pmin2;
         const double q_limit = k * (p - pmin2);
         return q > q_limit;
     }
-    const double y = (1. + 2. * beta) * q * q + mSq * (p + beta * pmax) * (p - pmax);
+    const double y = c.b2 * q * q + mSq * (p + c.bpmax) * (p - pmax);
     return y > 0;
 }
 
@@ -266,9 +269,9 @@ __device__ __forceinline__ void drucker_prager(const SceneConsts &c, double F[4]
     const double sh[2] = {f * (s2[0] - half), f * (s2[1] - half)};
 
     if (p < -c.dp_threshold_p || Jp_inv < 1) {
-        const double p_new = -c.dp_threshold_p;
-        const double Je_new = sqrt(-2. * p_new / c.kappa + 1.);
-        const double sq = sqrt(Je_new);
+        // p_new = -DP_threshold_p: Je_new and its root are scene constants
+        const double Je_new = c.je_tension;
+        const double sq = c.sq_je_tension;
         usvt(U, sq, sq, V, F);
         Jp_inv *= Je_new / Je;
     } else {
@@ -279,7 +282,7 @@ __device__ __forceinline__ void drucker_prager(const SceneConsts &c, double F[4]
         } else {
             const double q_from_dp = (p + c.dp_threshold_p) * c.tan_phi;
             const double pmin = -c.its;
-            const double q_fs = 2 * sqrt((pmax - p) * (p - pmin)) * qmax / (pmax - pmin);
+            const double q_fs = 2 * sqrt((pmax - p) * (p - pmin)) * qmax / c.dp;
             q_n_1 = fmin(q_fs, q_from_dp);
         }
         if (q >= q_n_1) {

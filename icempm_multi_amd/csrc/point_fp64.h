@@ -125,7 +125,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
             column(i, ax, ay, bx, by_);
         }
     }
-    const double s4 = 4. * c.hinv;
+    const double s4 = c.s4;
     v[0] = vx;
     v[1] = vy;
     B[0] = s4 * c00;
