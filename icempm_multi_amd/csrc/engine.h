@@ -19,6 +19,10 @@
 namespace icempm {
 
 
+// The per-scene constants the kernels read, derived from the reference's
+// SimParams (parameters_sim.cpp:128-165) plus the invariants of the point loop.
+void scene_consts(const icempm_simparams &P, SceneConsts &c);
+
 enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kNumCls };
 
 struct SceneHost {
@@ -123,7 +127,6 @@ private:
     int collect_timings();
     void record(hipEvent_t e);
     void invalidate_graphs();
-    int frac_bits_for(const icempm_simparams &P, int &origin) const;
 
     int device_ = 0, precision_ = 0;
     hipStream_t stream_ = nullptr;

@@ -100,16 +100,6 @@ void Engine::clock(int scene, double *t, int *step) const
     *step = scenes_[scene].sim_step;
 }
 
-int Engine::frac_bits_for(const icempm_simparams &P, int &origin) const
-{
-    const int halo = P.GridHaloSize;
-    const int ncols = (scenes_.empty() ? P.GridXTotal : P.GridXTotal) + 2 * halo;
-    origin = halo + 2;
-    const long long span = std::max<long long>(ncols + 4, P.GridY + 4);
-    // fx = frac * 2^-fb must stay exact in FP32 (frac < 1.5 * 2^fb): fb <= 23
-    return std::min(23, 32 - bits_for(span));
-}
-
 int Engine::add_scene(const icempm_simparams *prms, int cap_pts)
 {
     if (cap_pts <= 0) return fail(ICEMPM_ERR_ARGUMENT, "add_scene: capacity must be positive");
@@ -170,12 +160,20 @@ int Engine::set_grid_span(int scene, int xoff, int gx)
     return ICEMPM_OK;
 }
 
-int Engine::build_scene_dev(int i, SceneDev &d) const
+// Fixed-point position map of the mixed form: x/h = xq * 2^-fb - origin.
+static int frac_bits_for(const icempm_simparams &P, int &origin)
 {
-    const SceneHost &s = scenes_[i];
-    const icempm_simparams &P = s.prms;
-    std::memset(&d, 0, sizeof d);
-    SceneConsts &c = d.c;
+    const int halo = P.GridHaloSize;
+    const int ncols = P.GridXTotal + 2 * halo;
+    origin = halo + 2;
+    const long long span = std::max<long long>(ncols + 4, P.GridY + 4);
+    // fx = frac * 2^-fb must stay exact in FP32 (frac < 1.5 * 2^fb): fb <= 23
+    return std::min(23, 32 - bits_for(span));
+}
+
+void scene_consts(const icempm_simparams &P, SceneConsts &c)
+{
+    std::memset(&c, 0, sizeof c);
     c.gridX = P.GridXTotal;
     c.gridY = P.GridY;
     c.halo = P.GridHaloSize;
@@ -258,6 +256,14 @@ int Engine::build_scene_dev(int i, SceneDev &d) const
         f.b2 = (float)(1. + 2. * c.beta);
         f.bpmax = (float)(c.beta * pmax);
     }
+}
+
+int Engine::build_scene_dev(int i, SceneDev &d) const
+{
+    const SceneHost &s = scenes_[i];
+    std::memset(&d, 0, sizeof d);
+    scene_consts(s.prms, d.c);
+    const SceneConsts &c = d.c;
     d.pt_off = s.pt_off;
     d.n_pts = s.n_pts;
     d.cap_pts = s.cap_pts;

@@ -16,7 +16,8 @@ checked:
 - keys equal the oracle's on a seeded sample of points.
 
 The FP64 form matches the FP64 oracle (1e-8 of each field's scale after 1000
-substeps) on the three smallest blocks, the largest block (445k points) and
+substeps, plus 1e-12 for v and C, which stay at round-off size on floes at
+rest) on the three smallest blocks, the largest block (445k points) and
 the smallest floe field (1.3M points).  The mixed form stays within 1e-4
 relative position error of the FP64 oracle after 1000 substeps on the same
 scenes.
@@ -134,11 +135,17 @@ def check_scene(b, i, P, n, mid, final, precision, rng):
         assert keys[k] == ref, (i, k, j, keys[k], ref)
 
 
+# Absolute floors of the comparison: floes that stay at rest keep v and C at
+# round-off size (1e-14), where a bar relative to the field's own size is
+# meaningless.  1e-12 m/s and 1e-12 1/s are far below any physical motion here.
+FLOOR = {"v": 1e-12, "C": 1e-12}
+
+
 def assert_matches_oracle(out, ref, n, what):
     for name, (a, z) in FIELDS.items():
         s = max(np.abs(ref[a:z, :n]).max(), 1e-300)
         d = np.abs(out[a:z, :n] - ref[a:z, :n]).max()
-        assert d <= 1e-8 * s, f"{what} {name}: max diff {d:.3e} scale {s:.3e}"
+        assert d <= 1e-8 * s + FLOOR.get(name, 0.0), f"{what} {name}: max diff {d:.3e} scale {s:.3e}"
     mism = np.count_nonzero(out[0, :n].view(np.int64) != ref[0, :n].view(np.int64))
     assert mism == 0, f"{what}: {mism} utility words differ"
 
