@@ -23,7 +23,15 @@ namespace icempm {
 // SimParams (parameters_sim.cpp:128-165) plus the invariants of the point loop.
 void scene_consts(const icempm_simparams &P, SceneConsts &c);
 
-enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kNumCls };
+enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kClsDeferred, kNumCls };
+
+// Workgroups of the launch that handles the points a tile launch set aside.
+constexpr int kDeferredBlocks = 8;
+
+// Grid buffers per scene.  A stepping call rotates through three: the tile
+// launch of substep k reads buffer k (momenta), accumulates into k+1 and
+// clears k+2 (mod 3).
+constexpr int kGridBuffers = 3;
 
 struct SceneHost {
     icempm_simparams prms{};
@@ -34,9 +42,9 @@ struct SceneHost {
     long long cells_off = 0;
     long long grid_elems = 0; // gpitch
     long long seg_off = 0;    // first segment flag of the scene (per buffer)
-    void *grid[2] = {nullptr, nullptr};
+    void *grid[3] = {nullptr, nullptr, nullptr};
     double *ind_force = nullptr;
-    bool clean[2] = {true, true};
+    bool clean[3] = {true, true, true};   // the buffer holds zeros (and no segment flags)
     double sim_time = 0;      // SimulationTime
     int sim_step = 0;         // SimulationStep
 };
@@ -44,7 +52,7 @@ struct SceneHost {
 class Engine {
 public:
     // Units of a substep call (see emit_units in engine.hip).
-    enum UnitKind { kUnitHead = 0, kUnitFused, kUnitTail, kUnitUnfused, kUnitDirect };
+    enum UnitKind { kUnitHead = 0, kUnitChain, kUnitTail, kUnitUnfused, kUnitDirect };
 
     Engine(int device, int precision);
     ~Engine();
@@ -121,6 +129,8 @@ private:
                   std::vector<GraphExec *> &replayed);
     int read_graph_events(const GraphExec &gx);
     int zero_grid(int scene, int buf);
+    int start_buffer(bool fused, bool direct, int &b);
+    void mark_buffers(bool fused, bool direct, int b, int nsub);
     int record(int cls, bool timed, hipError_t (*)(void *), void *);
     void timed_begin(bool timed);
     void timed_end(bool timed, int cls);
@@ -144,11 +154,12 @@ private:
     SceneClock *d_clocks_ = nullptr;
     unsigned *d_err_ = nullptr;
     int *d_tile_queue_ = nullptr;
-    int *d_dirty_ = nullptr;   // 2 * n_segs_ segment flags
+    int *d_dirty_ = nullptr;   // kGridBuffers * n_segs_ segment flags
+    int *d_defer_ = nullptr;   // deferred-point list: 2 * n_pts_space_ ints and 64 counters
     int clock_slot_ = 0;       // slot of d_clocks_ (2 x nscenes) holding the current clock
     // workgroups of a grid update over n runs: a run per wave, persistent beyond 8 per CU
     int tile_blocks(int mode);      // workgroups of a tile launch: the resident count
-    int resident_[4] = {0, 0, 0, 0};   // per tile mode; reset when the tile shape changes
+    int resident_[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // per tile mode; reset when the tile shape changes
     int update_blocks(int n_runs) const { return std::max(1, std::min((n_runs + 3) / 4, n_cus_ * 8)); }
     long long n_segs_ = 0;
     NodeChunk *d_chunks_ = nullptr;
@@ -189,7 +200,7 @@ private:
         std::vector<int> cls;
     };
     typedef std::array<int, 7> GraphKey;
-    static constexpr int kMaxChunk = 64;      // units per graph: at most 128 kernel nodes (3 x 64 unfused)
+    static constexpr int kMaxChunk = 64;      // units per graph: at most 192 kernel nodes (3 per unfused unit)
     static constexpr size_t kMaxGraphs = 256; // the cache is cleared beyond this
     std::map<GraphKey, GraphExec> graphs_;
     int pts_flip_ = 0;                        // which buffer of the pair pts_[0] is (the sort swaps them)

@@ -57,7 +57,7 @@ Engine::~Engine()
     if (ev_a_) hipEventDestroy(ev_a_);
     if (ev_b_) hipEventDestroy(ev_b_);
     void *bufs[] = {pts_[0], pts_[1], grid_mem_, ind_force_mem_, d_scenes_, d_clocks_, d_err_, d_chunks_,
-                    bin_mem_, tile_mem_, d_tile_queue_, d_dirty_};
+                    bin_mem_, tile_mem_, d_tile_queue_, d_dirty_, d_defer_};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (stream_) hipStreamDestroy(stream_);
@@ -267,8 +267,7 @@ int Engine::build_scene_dev(int i, SceneDev &d) const
     d.pt_off = s.pt_off;
     d.n_pts = s.n_pts;
     d.cap_pts = s.cap_pts;
-    d.grid_off[0] = (char *)s.grid[0] - (char *)grid_mem_;
-    d.grid_off[1] = (char *)s.grid[1] - (char *)grid_mem_;
+    for (int b = 0; b < kGridBuffers; b++) d.grid_off[b] = (char *)s.grid[b] - (char *)grid_mem_;
     d.gpitch = s.grid_elems;
     d.cell_off = s.cells_off;
     d.ncells = c.ncols * c.gridY;
@@ -354,7 +353,15 @@ int Engine::commit()
         for (int i = 0; i < nscenes(); i++) { scenes_[i].pt_off = offs[i]; scenes_[i].n_pts = 0; }
     }
 
-    // grids: [buffer 0 of all scenes][buffer 1 of all scenes], 24 bytes per node each
+    // deferred-point list of the kTileFromMomenta launches: a point index and
+    // a scene per point of the point space, and two counters (zero)
+    if (!points_layout_same || !d_defer_) {
+        if (d_defer_) { hipFree(d_defer_); d_defer_ = nullptr; }
+        HIPCHK(hipMalloc(&d_defer_, sizeof(int) * (2 * n_pts_space_ + 64)));
+        HIPCHK(hipMemsetAsync(d_defer_ + 2 * n_pts_space_, 0, sizeof(int) * 64, stream_));
+    }
+
+    // grids: [buffer 0 of all scenes][buffer 1 ...][buffer 2 ...], 24 bytes per node each
     if (grid_mem_) { hipFree(grid_mem_); grid_mem_ = nullptr; }
     long long gelems = 0, cells = 0, forces = 0;
     std::vector<long long> goff(nscenes()), coff(nscenes()), foff(nscenes());
@@ -374,8 +381,8 @@ int Engine::commit()
     // cell keys, bin counts and cell starts are 32-bit
     if (cells >= (1LL << 31) - 1) return fail(ICEMPM_ERR_CAPACITY, "grid cells of all scenes exceed 2^31");
     const size_t buf_bytes = (size_t)gelems * eb;
-    HIPCHK(hipMalloc(&grid_mem_, 2 * buf_bytes));
-    HIPCHK(hipMemsetAsync(grid_mem_, 0, 2 * buf_bytes, stream_));
+    HIPCHK(hipMalloc(&grid_mem_, kGridBuffers * buf_bytes));
+    HIPCHK(hipMemsetAsync(grid_mem_, 0, kGridBuffers * buf_bytes, stream_));
     // segment flags (grid_elems is a multiple of the segment size); each scene's
     // range starts 16-byte aligned for the zero-fill kernel
     n_segs_ = 0;
@@ -384,17 +391,16 @@ int Engine::commit()
         n_segs_ += round_up(scenes_[i].grid_elems >> kSegShift, 4);
     }
     if (d_dirty_) { hipFree(d_dirty_); d_dirty_ = nullptr; }
-    HIPCHK(hipMalloc(&d_dirty_, sizeof(int) * 2 * std::max<long long>(n_segs_, 1)));
-    HIPCHK(hipMemsetAsync(d_dirty_, 0, sizeof(int) * 2 * std::max<long long>(n_segs_, 1), stream_));
+    HIPCHK(hipMalloc(&d_dirty_, sizeof(int) * kGridBuffers * std::max<long long>(n_segs_, 1)));
+    HIPCHK(hipMemsetAsync(d_dirty_, 0, sizeof(int) * kGridBuffers * std::max<long long>(n_segs_, 1), stream_));
     if (ind_force_mem_) hipFree(ind_force_mem_);
     HIPCHK(hipMalloc(&ind_force_mem_, sizeof(double) * std::max<long long>(forces, 1)));
     HIPCHK(hipMemsetAsync(ind_force_mem_, 0, sizeof(double) * std::max<long long>(forces, 1), stream_));
     for (int i = 0; i < nscenes(); i++) {
-        scenes_[i].grid[0] = (char *)grid_mem_ + goff[i] * eb;
-        scenes_[i].grid[1] = (char *)grid_mem_ + buf_bytes + goff[i] * eb;
+        for (int b = 0; b < kGridBuffers; b++) scenes_[i].grid[b] = (char *)grid_mem_ + b * buf_bytes + goff[i] * eb;
         scenes_[i].cells_off = coff[i];
         scenes_[i].ind_force = ind_force_mem_ + foff[i];
-        scenes_[i].clean[0] = scenes_[i].clean[1] = true;
+        for (int b = 0; b < kGridBuffers; b++) scenes_[i].clean[b] = true;
     }
 
     // tables
@@ -499,6 +505,9 @@ StepArgs Engine::args() const
     a.dyn_tiles = dyn_tiles_;
     a.tile_queue = d_tile_queue_;
     a.dirty = d_dirty_;
+    a.defer_pts = d_defer_;
+    a.defer_scene = d_defer_ ? d_defer_ + n_pts_space_ : nullptr;
+    a.defer_count = d_defer_ ? d_defer_ + 2 * n_pts_space_ : nullptr;
     a.dirty_stride = n_segs_;
     return a;
 }
@@ -738,7 +747,7 @@ int Engine::reset_grid(int scene)
 // 106.0 us with 8, 103.6 us with 4, its residency, 133 us with 2).
 int Engine::tile_blocks(int mode)
 {
-    int &r = resident_[mode & 3];
+    int &r = resident_[mode & 7];
     if (r <= 0) {
         r = LAUNCH_PREC(tiles_resident_per_cu, args(), mode);
         if (r <= 0) r = 8;   // the runtime could not tell: several generations
@@ -752,7 +761,7 @@ int Engine::p2g(int scene)
     const StepArgs a = args();
     timed_begin(true);
     if (bins_valid_ && nscenes() == 1) {
-        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, 0, 0, 0, tile_blocks(kTileP2G), stream_));
+        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileP2G, 0, 0, 0, 0, tile_blocks(kTileP2G), stream_));
     } else {
         HIPCHK(LAUNCH_PREC(launch_p2g_direct, a, scene, scenes_[scene].n_pts, 0, stream_));
     }
@@ -801,7 +810,7 @@ int Engine::g2p(int scene, int recordPQ)
     const StepArgs a = args();
     timed_begin(true);
     if (bins_valid_ && nscenes() == 1) {
-        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, 0, 0, recordPQ ? 1 : 0, tile_blocks(kTileG2P), stream_));
+        HIPCHK(LAUNCH_PREC(launch_tiles, a, kTileG2P, 0, 0, 0, recordPQ ? 1 : 0, tile_blocks(kTileG2P), stream_));
     } else {
         HIPCHK(LAUNCH_PREC(launch_g2p_direct, a, scene, scenes_[scene].n_pts, 0, recordPQ ? 1 : 0, stream_));
     }
@@ -870,7 +879,7 @@ int Engine::get_position_map(int scene, int *fb, int *ox, int *oy)
 int Engine::get_grid(int scene, int buf, double *out, long long n_nodes)
 {
     RC(ensure_committed());
-    if (buf < 0 || buf > 1) return fail(ICEMPM_ERR_ARGUMENT, "get_grid: buffer 0 or 1");
+    if (buf < 0 || buf >= kGridBuffers) return fail(ICEMPM_ERR_ARGUMENT, "get_grid: buffer 0, 1 or 2");
     const SceneHost &s = scenes_[scene];
     const long long nodes = (long long)(s.prms.GridXTotal + 2 * s.prms.GridHaloSize) * s.prms.GridY;
     if (n_nodes != nodes) return fail(ICEMPM_ERR_ARGUMENT, "get_grid: node count mismatch");
@@ -961,13 +970,16 @@ int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *ce
 // frame stepping
 // ---------------------------------------------------------------------------
 // A call of nsub substeps is a sequence of segments of one kind of unit:
-//   binned, fused:   Head [P2G(b)], nsub-1 x Fused [update(b), G2P(b)+P2G(b^1)],
-//                    Tail [update(b), G2P(b)]
-//   binned, unfused: nsub x Unfused [P2G(b), update(b), G2P(b)]
+//   binned, fused:   Head [P2G(b)], nsub-1 x Chain, Tail
+//     Chain [grid update + G2P from grid b, P2G into b+1, clear b+2; the
+//            points set aside], then b = b+1 (mod 3)
+//     Tail  [grid update + G2P from grid b, clear b+2; the points set aside]
+//   binned, unfused: nsub x Unfused [P2G(b), update(b), G2P(b)], b = b^1 (0, 1)
 //   unbinned:        nsub x Direct [zero, P2G direct, update, G2P direct]
-// Every unit but the head advances the clock slot; the Fused and Unfused units
-// also swap the grid buffers.  The update of a substep reads clock slot cs and
-// writes the advanced clock to cs^1, which the later launches of the substep read.
+// Every unit but the head advances the clock slot.  The tile launch of a
+// Chain or Tail unit forms the node velocities of grid b from its momenta (the
+// grid update folded in), reading clock slot cs and writing the advanced
+// clock to cs^1, which the later launches of the substep read.
 int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sample, int k0)
 {
     StepArgs a = args();
@@ -984,9 +996,23 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
         switch (kind) {
         case kUnitHead:
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, tile_blocks(kTileP2G), stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, b, -1, tile_blocks(kTileP2G), stream_));
             timed_end(timed, kClsP2G);
             continue;                  // part of substep 0: no clock or buffer change
+        case kUnitChain:
+        case kUnitTail: {
+            const int mode = kind == kUnitChain ? kTileChain : kTileTail;
+            const int w = kind == kUnitChain ? (b + 1) % kGridBuffers : b, z = (b + 2) % kGridBuffers;
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_tiles, au, mode, b, w, z, -1, tile_blocks(mode), stream_));
+            timed_end(timed, kind == kUnitChain ? kClsFused : kClsG2P);
+            timed_begin(timed);
+            HIPCHK(LAUNCH_PREC(launch_deferred, au, mode, b, w, kDeferredBlocks, stream_));
+            timed_end(timed, kClsDeferred);
+            b = w;
+            cs ^= 1;
+            continue;
+        }
         case kUnitDirect:
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++) {
@@ -1010,28 +1036,20 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
             timed_end(timed, kClsG2P);
             cs ^= 1;
             continue;
-        case kUnitUnfused:
+        default:   // kUnitUnfused
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, -1, tile_blocks(kTileP2G), stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, b, -1, tile_blocks(kTileP2G), stream_));
             timed_end(timed, kClsP2G);
-            break;
-        default:
-            break;
-        }
-        timed_begin(timed);
-        HIPCHK(LAUNCH_PREC(launch_update, au, b, 1, 1, nb_upd, stream_));
-        timed_end(timed, kClsUpdate);
-        if (kind == kUnitFused) {
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileFused, b, b ^ 1, -1, tile_blocks(kTileFused), stream_));
-            timed_end(timed, kClsFused);
-        } else {
+            HIPCHK(LAUNCH_PREC(launch_update, au, b, 1, 1, nb_upd, stream_));
+            timed_end(timed, kClsUpdate);
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, -1, tile_blocks(kTileG2P), stream_));
+            HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, b, -1, tile_blocks(kTileG2P), stream_));
             timed_end(timed, kClsG2P);
+            cs ^= 1;
+            b ^= 1;
+            continue;
         }
-        cs ^= 1;
-        b ^= 1;
     }
     return ICEMPM_OK;
 }
@@ -1041,7 +1059,8 @@ static void advance_units(int kind, int n, int &b, int &cs)
 {
     if (kind == Engine::kUnitHead) return;
     cs ^= n & 1;
-    if (kind != Engine::kUnitDirect) b ^= n & 1;
+    if (kind == Engine::kUnitChain) b = (b + n) % kGridBuffers;
+    else if (kind == Engine::kUnitUnfused) b ^= n & 1;
 }
 
 // The graph of n units of a kind from (b, cs), captured on first use.  The
@@ -1061,9 +1080,7 @@ int Engine::chunk_graph(int kind, int n, int b, int cs, int tmode, bool first_ti
         // from the pool (emptied first) and keeps them.
         if (tmode) RC(collect_timings());
         // workgroup counts are runtime queries: make them before the capture
-        tile_blocks(kTileG2P);
-        tile_blocks(kTileP2G);
-        tile_blocks(kTileFused);
+        for (int m : {(int)kTileG2P, (int)kTileP2G, (int)kTileTail, (int)kTileChain}) tile_blocks(m);
         HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         capturing_ = true;
         // first_timed with sampling: unit 0 of the chunk is a sampled substep
@@ -1143,6 +1160,51 @@ int Engine::run_units(int kind, int n, int &b, int &cs, bool use_graph, int tmod
     return ICEMPM_OK;
 }
 
+// The first grid buffer of a call.  Fused: w0 with buffers w0 and w0+1 clean
+// in every scene (the head accumulates into w0, the first chain launch into
+// w0+1), buffers zeroed if no such pair exists.  Unfused: a clean buffer 0 or 1
+// (the grid update of each substep clears the other).  Unbinned: 0.
+int Engine::start_buffer(bool fused, bool direct, int &b)
+{
+    b = 0;
+    if (direct) return ICEMPM_OK;
+    auto all_clean = [&](int k) {
+        for (auto &s : scenes_)
+            if (!s.clean[k]) return false;
+        return true;
+    };
+    if (fused) {
+        for (int w = 0; w < kGridBuffers; w++)
+            if (all_clean(w) && all_clean((w + 1) % kGridBuffers)) { b = w; return ICEMPM_OK; }
+        for (int i = 0; i < nscenes(); i++)
+            for (int k = 0; k < 2; k++)
+                if (!scenes_[i].clean[k]) RC(zero_grid(i, k));
+        return ICEMPM_OK;
+    }
+    if (all_clean(0)) return ICEMPM_OK;
+    if (all_clean(1)) { b = 1; return ICEMPM_OK; }
+    for (int i = 0; i < nscenes(); i++) RC(zero_grid(i, 0));
+    return ICEMPM_OK;
+}
+
+// Clean grid buffers after a call of nsub substeps from buffer b (see emit_units).
+void Engine::mark_buffers(bool fused, bool direct, int b, int nsub)
+{
+    for (auto &s : scenes_) {
+        if (direct) {
+            s.clean[0] = false;
+        } else if (!fused) {
+            const int last = b ^ ((nsub - 1) & 1);
+            s.clean[last] = false;
+            s.clean[last ^ 1] = true;
+        } else {
+            // only the buffer the tail read holds data; the others were cleared
+            const int last = (b + nsub - 1) % kGridBuffers;
+            for (int k = 0; k < kGridBuffers; k++) s.clean[k] = k != last;
+        }
+    }
+}
+
 // The units of a call of nsub substeps (see emit_units), without graphs.
 int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed, int sample)
 {
@@ -1155,7 +1217,7 @@ int Engine::launch_substeps(int nsub, int start_buf, bool fused, bool direct, bo
         RC(run_units(kUnitUnfused, nsub, b, cs, false, tmode, 0, none));
     } else {
         RC(run_units(kUnitHead, 1, b, cs, false, tmode, 0, none));
-        RC(run_units(kUnitFused, nsub - 1, b, cs, false, tmode, 0, none));
+        RC(run_units(kUnitChain, nsub - 1, b, cs, false, tmode, 0, none));
         RC(run_units(kUnitTail, 1, b, cs, false, tmode, nsub - 1, none));
     }
     clock_slot_ = cs;
@@ -1174,16 +1236,7 @@ int Engine::step(int nsub, int flags)
     const int tmode = timed ? ((flags & ICEMPM_STEP_SAMPLED) ? 2 : 1) : 0;
 
     int start_buf = 0;
-    if (!direct) {
-        bool c0 = true, c1 = true;
-        for (auto &s : scenes_) { c0 = c0 && s.clean[0]; c1 = c1 && s.clean[1]; }
-        if (c0) start_buf = 0;
-        else if (c1) start_buf = 1;
-        else {
-            for (int i = 0; i < nscenes(); i++) RC(zero_grid(i, 0));
-            start_buf = 0;
-        }
-    }
+    RC(start_buffer(fused, direct, start_buf));
     HIPCHK(hipEventRecord(ev_a_, stream_));
     std::vector<GraphExec *> replayed;
     int b = start_buf, cs = clock_slot_;
@@ -1193,7 +1246,7 @@ int Engine::step(int nsub, int flags)
         RC(run_units(kUnitUnfused, nsub, b, cs, use_graph, tmode, 0, replayed));
     } else {
         RC(run_units(kUnitHead, 1, b, cs, use_graph, tmode, 0, replayed));
-        RC(run_units(kUnitFused, nsub - 1, b, cs, use_graph, tmode, 0, replayed));
+        RC(run_units(kUnitChain, nsub - 1, b, cs, use_graph, tmode, 0, replayed));
         RC(run_units(kUnitTail, 1, b, cs, use_graph, tmode, nsub - 1, replayed));
     }
     clock_slot_ = cs;
@@ -1207,14 +1260,8 @@ int Engine::step(int nsub, int flags)
         RC(collect_timings());
     }
 
+    mark_buffers(fused, direct, start_buf, nsub);
     for (auto &s : scenes_) {
-        if (direct) {
-            s.clean[0] = false;
-        } else {
-            const int last = start_buf ^ ((nsub - 1) & 1);
-            s.clean[last] = false;
-            s.clean[last ^ 1] = true;
-        }
         for (int k = 0; k < nsub; k++) s.sim_time += s.prms.InitialTimeStep;
         s.sim_step += nsub;
     }
@@ -1272,10 +1319,11 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
         if (set_tile_shape(cf[0], cf[1], margin) != ICEMPM_OK) return ICEMPM_OK;
         RC(step(1, 0));   // warm-up
         std::vector<GraphExec *> none;
-        int b = scenes_[0].clean[0] ? 0 : 1, cs = clock_slot_;
+        int b = 0, cs = clock_slot_;
+        RC(start_buffer(true, false, b));
         RC(run_units(kUnitHead, 1, b, cs, false, 0, 0, none));
         HIPCHK(hipEventRecord(ev_a_, stream_));
-        RC(run_units(kUnitFused, n_trials, b, cs, false, 0, 0, none));
+        RC(run_units(kUnitChain, n_trials, b, cs, false, 0, 0, none));
         HIPCHK(hipEventRecord(ev_b_, stream_));
         RC(run_units(kUnitTail, 1, b, cs, false, 0, n_trials + 1, none));
         clock_slot_ = cs;
@@ -1284,8 +1332,7 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
         ms /= n_trials;
         HIPCHK(hipMemcpyAsync(pts_[0], pts_[1], pts_bytes, hipMemcpyDeviceToDevice, stream_));
         for (int i = 0; i < nscenes(); i++) {
-            RC(zero_grid(i, 0));
-            RC(zero_grid(i, 1));
+            for (int k = 0; k < kGridBuffers; k++) RC(zero_grid(i, k));
             scenes_[i].sim_time = drifted[i].sim_time;
             scenes_[i].sim_step = drifted[i].sim_step;
         }
@@ -1322,10 +1369,7 @@ int Engine::autotune(int n_trials, float *out_ms, int n_out, int *chosen)
     for (int i = 0; i < nscenes(); i++) {
         scenes_[i].sim_time = saved[i].sim_time;
         scenes_[i].sim_step = saved[i].sim_step;
-        if (rc == ICEMPM_OK) {
-            rc = zero_grid(i, 0);
-            if (rc == ICEMPM_OK) rc = zero_grid(i, 1);
-        }
+        for (int k = 0; k < kGridBuffers && rc == ICEMPM_OK; k++) rc = zero_grid(i, k);
     }
     if (best_i < 0 || rc != ICEMPM_OK) {
         lane_spread_ = ls0;

@@ -8,15 +8,29 @@ namespace icempm {
 
 // kernels_fp64.hip / kernels_mixed.hip share these signatures.
 // Tile kernels: mode bit 1 = G2P from grid[rbuf], bit 2 = P2G into grid[wbuf].
-enum TileMode { kTileG2P = 1, kTileP2G = 2, kTileFused = 3 };
+// Bit 4 folds the grid update in: G2P stages node velocities computed from the
+// momenta of grid[rbuf] (instead of reading velocities a grid update wrote),
+// and the launch also clears the flagged segments of grid[zbuf], adds the
+// indenter force of the nodes inside the indenter and advances the clocks.
+enum TileMode {
+    kTileG2P = 1,
+    kTileP2G = 2,
+    kTileFused = 3,
+    kTileFromMomenta = 4,
+    kTileTail = kTileG2P | kTileFromMomenta,     // last G2P of a call
+    kTileChain = kTileFused | kTileFromMomenta,  // grid update + G2P(n) + P2G(n+1)
+};
 
 namespace fp64 {
 // record_mode: 1/0 = write P, Q or not; -1 = decide from the scene clock.
-hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
+hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbuf, int record_mode, int n_blocks,
                         hipStream_t s);
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
+// The points a kTileFromMomenta launch of `mode` set aside (their stencil left
+// the LDS window), from grid[rbuf] (momenta) into grid[wbuf].
+hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 // Workgroups of the tile kernel for this mode and window that one CU holds at
 // once (registers and LDS); 0 if the runtime cannot tell.
@@ -25,11 +39,14 @@ int tiles_resident_per_cu(const StepArgs &a, int mode);
 
 namespace mixed {
 // record_mode: 1/0 = write P, Q or not; -1 = decide from the scene clock.
-hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
+hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbuf, int record_mode, int n_blocks,
                         hipStream_t s);
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
+// The points a kTileFromMomenta launch of `mode` set aside (their stencil left
+// the LDS window), from grid[rbuf] (momenta) into grid[wbuf].
+hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 // Workgroups of the tile kernel for this mode and window that one CU holds at
 // once (registers and LDS); 0 if the runtime cannot tell.

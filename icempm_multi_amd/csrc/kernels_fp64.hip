@@ -27,6 +27,118 @@ namespace icempm {
 namespace fp64 {
 
 // ---------------------------------------------------------------------------
+// Parts of the grid update that the tile kernels take over (kTileFromMomenta).
+// ---------------------------------------------------------------------------
+// Clears the flagged node segments of grid buffer zb of every scene, and
+// their flags.  zb is the buffer two substeps back: no launch reads it any
+// more, and the next launch accumulates into it.  Wave w of the launch takes
+// runs w, w + waves, ... of the run table (a run: up to 64 segments of 64
+// nodes; one flag per lane, a ballot picks the flagged ones).
+__device__ __forceinline__ void clear_flagged(const StepArgs &a, int zb)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (kBlock / 64);
+    for (int r = wave; r < a.n_chunks; r += n_waves) {
+        const NodeChunk run = uniform_copy(a.chunks[r]);
+        const SceneDev &S = a.scenes[run.scene];
+        glb_d *g = (glb_d *)(a.grid_mem + S.grid_off[zb]);
+        const long long gp = S.gpitch;
+        int *fl = a.dirty + zb * a.dirty_stride + S.seg_off + run.seg0;
+        const int f = lane < run.nseg ? fl[lane] : 0;
+        unsigned long long mb = __ballot(f != 0);
+        const long long node0 = (long long)run.seg0 << kSegShift;
+        while (mb) {
+            const int sg = __builtin_ctzll(mb);
+            mb &= mb - 1;
+            const long long idx = node0 + ((long long)sg << kSegShift) + lane;
+            g[idx] = 0;
+            g[gp + idx] = 0;
+            g[2 * gp + idx] = 0;
+        }
+        if (f) fl[lane] = 0;
+    }
+}
+
+// The indenter's reaction of the substep (kernels.cuh:265-287): every node
+// of scene sc inside the indenter circle that holds mass in grid rb adds its
+// force to the scene's bins.  The tiles form the same node velocities again
+// where they stage them (node_velocity without the force).  One workgroup per
+// scene; it visits the bounding box of the circle, clipped to the grid.
+__device__ __forceinline__ void indenter_force_nodes(const StepArgs &a, int sc, int rb)
+{
+    const SceneDev &S = a.scenes[sc];
+    const SceneConsts c = uniform_copy(S.c);
+    if (!(c.ind_rsq > 0)) return;
+    const SceneClock clk = next_clock(c, uniform_copy(a.clocks[sc]));
+    const double r = sqrt(c.ind_rsq);
+    const int gx0 = max((int)floor((clk.ind_x - r) * c.hinv) - 1, c.xoff - c.halo);
+    const int gx1 = min((int)ceil((clk.ind_x + r) * c.hinv) + 1, c.xoff - c.halo + c.ncols - 1);
+    const int gy0 = max((int)floor((clk.ind_y - r) * c.hinv) - 1, 0);
+    const int gy1 = min((int)ceil((clk.ind_y + r) * c.hinv) + 1, c.gridY - 1);
+    if (gx1 < gx0 || gy1 < gy0) return;
+    const int ny = gy1 - gy0 + 1;
+    const long long nn = (long long)(gx1 - gx0 + 1) * ny;
+    const glb_d *g = (const glb_d *)(a.grid_mem + S.grid_off[rb]);
+    const long long gp = S.gpitch;
+    double *force = a.force_mem + S.force_off;
+    for (long long k = threadIdx.x; k < nn; k += kBlock) {
+        const int gix = gx0 + (int)(k / ny), giy = gy0 + (int)(k % ny);
+        const long long idx = (long long)(gix - c.xoff + c.halo) * c.gridY + giy;
+        const double m = g[idx];
+        if (m == 0) continue;
+        double vx, vy;
+        node_velocity(c, clk, force, gix, giy, m, g[gp + idx], g[2 * gp + idx], vx, vy);
+    }
+}
+
+// Entry k of the deferred-point list (a point whose G2P stencil left its
+// tile's LDS window): G2P from the momenta of grid rb with every node
+// velocity formed from global memory, the state stored as the tile kernel
+// stores it, and (p2g) P2G into grid wb with global atomics.
+__device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int rb, int wb, int k)
+{
+    const PtRef R{(double *)a.pts, a.pitch};
+    const long long pt = a.defer_pts[k];
+    const int sc = a.defer_scene[k];
+    const SceneDev &S = a.scenes[sc];
+    const SceneConsts &c = S.c;
+    const SceneClock clk = next_clock(c, a.clocks[sc]);
+    const bool record = (clk.step % c.update_every) == 0;
+    long long ud = as_ll(R.at(kLineUtility, pt));
+    double x, y, F[4], Jp, v[2], B[4], pp, qq;
+    load_state(R, pt, x, y, F, Jp);
+    unsigned e;
+    StressTerms terms;
+    terms.valid = false;
+    const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rb]);
+    if (!g2p_point(c, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, false, x, y, v, B, F, Jp, ud, pp, qq, e, terms)) {
+        atomicMax(&a.err[sc], e);
+        return;
+    }
+    if (!p2g) {
+        store_state(R, pt, x, y, v, F, B, Jp, ud, record, pp, qq);
+        return;
+    }
+    store_state_fused(R, pt, x, y, F, Jp, ud, record, pp, qq);
+    glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wb]);
+    const long long gp = S.gpitch;
+    const int gridY = c.gridY;
+    P2GStencil st;
+    const unsigned e2 = p2g_stencil(c, x, y, v, B, F, Jp, ud, terms, st);
+    if (e2) {
+        atomicMax(&a.err[sc], e2);
+        return;
+    }
+    st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
+        const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
+        glb_add(gw + gi, dm);
+        glb_add(gw + gp + gi, dpx);
+        glb_add(gw + 2 * gp + gi, dpy);
+        mark_dirty(a, S, wb, gi);
+    });
+}
+
+// ---------------------------------------------------------------------------
 // Binned tile kernel: G2P from grid[rbuf] and/or P2G into grid[wbuf].
 // 3 waves per SIMD (up to 168 VGPRs; the fused kernel fits with no spills).
 // What keeps it there: the scene invariants of the point loop (4/h, kappa/2,
@@ -39,12 +151,19 @@ namespace fp64 {
 // WHC is the window height (ty + 2 + 2*margin) when fixed at compile time for
 // the common tile shapes (0: read from the arguments).  A compile-time height
 // turns the stencil's LDS offsets into instruction immediates.
+//
+// With kTileFromMomenta (the chain of a stepping call) grid[rbuf] holds the
+// momenta of the last P2G: the staging forms each window node's velocity
+// (the grid update of kernels.cuh:164-301, per node), the launch clears
+// grid[zbuf] for the next P2G, adds the indenter force and writes the advanced
+// clocks to the other slot.  Without it grid[rbuf] holds velocities.
 template <int MODE, int WHC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) k_tiles(StepArgs a, int rbuf, int wbuf,
-                                                                                        int record_mode)
+                                                                                        int zbuf, int record_mode)
 {
     constexpr bool DO_G2P = (MODE & kTileG2P) != 0;
     constexpr bool DO_P2G = (MODE & kTileP2G) != 0;
+    constexpr bool FROM_M = (MODE & kTileFromMomenta) != 0;
     extern __shared__ double lds_raw[];
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
@@ -63,6 +182,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(double *)a.pts, a.pitch};
+
+    if (FROM_M) {
+        clear_flagged(a, zbuf);
+        for (int sc = blockIdx.x; sc < a.nscenes; sc += gridDim.x) indenter_force_nodes(a, sc, rbuf);
+        // the advanced clocks go to the other slot, which no block reads here
+        if (blockIdx.x == 0)
+            for (int sc = tid; sc < a.nscenes; sc += kBlock)
+                a.clocks_next[sc] = next_clock(a.scenes[sc].c, a.clocks[sc]);
+    }
 
     // Tile distribution.  Static: block b takes tiles b', b'+G, ... (b' the
     // XCD-contiguous remap of b).  Dynamic: each XCD (blockIdx.x % 8, the
@@ -106,9 +234,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
         int *const dflag = a.dirty + wbuf * a.dirty_stride + uniform_copy(S.seg_off);   // segment flags of wbuf
+        // the substep's clock: advanced here from the current slot, or already
+        // advanced by the grid update
+        SceneClock clk = uniform_copy(a.clocks[ti.scene]);
+        if (FROM_M) clk = next_clock(c, clk);
         bool record = false;
-        if (DO_G2P)
-            record = record_mode >= 0 ? record_mode != 0 : (a.clocks[ti.scene].step % c.update_every) == 0;
+        if (DO_G2P) record = record_mode >= 0 ? record_mode != 0 : (clk.step % c.update_every) == 0;
 
         // point runs of the tile's columns
         if (tid < a.tx) {
@@ -123,8 +254,36 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             run_pre[tid + 1] = (int)(s1 - s0);
         }
         // stage velocities / clear accumulators
+        if (DO_G2P && FROM_M) {
+            // velocities formed from the momenta: four window nodes per thread
+            // and pass, the loads of all four issued before any is used
+            constexpr int kU = 4;
+            for (int i0 = tid; i0 < WN; i0 += kU * kBlock) {
+                double m[kU], px[kU], py[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    const int i = i0 + u * kBlock;
+                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    const bool in = i < WN && col >= 0 && col < c.ncols && row >= 0 && row < gridY;
+                    const long long gi = (long long)col * gridY + row;
+                    m[u] = in ? (double)gr[gi] : 0.0;
+                    px[u] = in ? (double)gr[gp + gi] : 0.0;
+                    py[u] = in ? (double)gr[2 * gp + gi] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    const int i = i0 + u * kBlock;
+                    if (i >= WN) break;
+                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    double vx = 0, vy = 0;
+                    if (m[u] != 0) node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, m[u], px[u], py[u], vx, vy);
+                    vw[2 * i] = vx;
+                    vw[2 * i + 1] = vy;
+                }
+            }
+        }
         for (int i = tid; i < WN; i += kBlock) {
-            if (DO_G2P) {
+            if (DO_G2P && !FROM_M) {
                 const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 double vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
@@ -177,6 +336,43 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             pt = run_start[cidx] + (k - run_pre[cidx]);
             return true;
         };
+        // P2G of one point (kernels.cuh:59-121) into the LDS window, or into
+        // global memory for the nodes outside it (a point that drifted since
+        // the sort)
+        auto p2g_emit = [&](double x, double y, const double v[2], const double B[4], const double F[4], double Jp,
+                            long long ud, const StressTerms &terms) {
+            P2GStencil st;
+            const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, terms, st);
+            if (e) {
+                err = max(err, e);
+                return;
+            }
+            const int lx = st.bxc - wc0, ly = st.by - wr0;
+            if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
+                lds_d *const p0 = acc_lane + 3 * (lx * win_h + ly);
+                st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
+                    lds_d *const p = p0 + 3 * (i * win_h + j);
+                    lds_add(p, dm);
+                    lds_add(p + 1, dpx);
+                    lds_add(p + 2, dpy);
+                });
+            } else {
+                st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
+                    if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
+                        lds_d *const p = acc_lane + 3 * ((lx + i) * win_h + ly + j);
+                        lds_add(p, dm);
+                        lds_add(p + 1, dpx);
+                        lds_add(p + 2, dpy);
+                    } else {
+                        const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
+                        glb_add(gw + gi, dm);
+                        glb_add(gw + gp + gi, dpx);
+                        glb_add(gw + 2 * gp + gi, dpy);
+                        dflag[gi >> kSegShift] = 1;
+                    }
+                });
+            }
+        };
         // The state of the next point is loaded while the current one is
         // computed.  Its index is located one iteration earlier still: the
         // lookup reads LDS, and LDS reads wait for this wave's earlier LDS
@@ -210,8 +406,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             if (DO_G2P) {
                 double pp, qq;
                 unsigned e;
-                if (!g2p_point(c, gr, gp, vw, wc0, wr0, win_w, win_h, x, y, v, B, F, Jp, ud, pp, qq, e, terms)) {
-                    err = max(err, e);
+                // With velocities formed from momenta, a point whose G2P
+                // stencil leaves the LDS window (it drifted since the sort)
+                // goes to the deferred list, which k_deferred handles after
+                // this launch: forming node velocities from global memory
+                // here would add to the point loop's registers.
+                if (!g2p_point(c, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, x, y, v, B, F, Jp, ud, pp,
+                               qq, e, terms)) {
+                    if (e) {
+                        err = max(err, e);
+                    } else {
+                        const int slot = atomicAdd(a.defer_count, 1);
+                        a.defer_pts[slot] = (int)pt;
+                        a.defer_scene[slot] = ti.scene;
+                    }
                     continue;
                 }
                 if (DO_P2G) store_state_fused(R, pt, x, y, F, Jp, ud, record, pp, qq);
@@ -222,40 +430,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 #pragma unroll
                 for (int m = 0; m < 4; m++) B[m] = R.at(kLineBp00 + m, pt);
             }
-            if (DO_P2G) {
-                P2GStencil st;
-                const unsigned e = p2g_stencil(c, x, y, v, B, F, Jp, ud, terms, st);
-                if (e) {
-                    err = max(err, e);
-                    continue;
-                }
-                const int lx = st.bxc - wc0, ly = st.by - wr0;
-                if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
-                    lds_d *const p0 = acc_lane + 3 * (lx * win_h + ly);
-                    st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
-                        lds_d *const p = p0 + 3 * (i * win_h + j);
-                        lds_add(p, dm);
-                        lds_add(p + 1, dpx);
-                        lds_add(p + 2, dpy);
-                    });
-                } else {
-                    // the stencil leaves the window (a point that drifted since the sort)
-                    st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
-                        if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
-                            lds_d *const p = acc_lane + 3 * ((lx + i) * win_h + ly + j);
-                            lds_add(p, dm);
-                            lds_add(p + 1, dpx);
-                            lds_add(p + 2, dpy);
-                        } else {
-                            const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
-                            glb_add(gw + gi, dm);
-                            glb_add(gw + gp + gi, dpx);
-                            glb_add(gw + 2 * gp + gi, dpy);
-                            dflag[gi >> kSegShift] = 1;
-                        }
-                    });
-                }
-            }
+            if (DO_P2G) p2g_emit(x, y, v, B, F, Jp, ud, terms);
         }
         if (err) atomicMax(&a.err[ti.scene], err);
         if (DO_P2G) {
@@ -281,6 +456,34 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
 }
 
 // ---------------------------------------------------------------------------
+// The points a kTileFromMomenta launch set aside (their G2P stencil left the
+// tile's LDS window), handled by the launch that follows it: the point list
+// needs every workgroup of that launch to have finished.  A few workgroups
+// suffice (points drift out of their window rarely); the last to finish
+// empties the list.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_deferred(StepArgs a, int p2g, int rbuf, int wbuf)
+{
+    const int n = *a.defer_count;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        deferred_point(a, p2g != 0, rbuf, wbuf, k);
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(a.defer_count + 1, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        a.defer_count[0] = 0;
+        a.defer_count[1] = 0;
+    }
+}
+
+hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s)
+{
+    if (n_blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_deferred, dim3(n_blocks), dim3(kBlock), 0, s, a, (mode & kTileP2G) ? 1 : 0, rbuf, wbuf);
+    return hipGetLastError();
+}
+
 size_t tile_lds_bytes(const StepArgs &a, int mode)
 {
     const size_t WN = (size_t)a.win_w * a.win_h;
@@ -292,30 +495,31 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
 
 // window heights compiled in: ty = 16, 32, 64 with a drift margin of 1 or 0
 template <int MODE>
-static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int record_mode, int n_blocks, size_t lds,
-                                  hipStream_t s)
+static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int zbuf, int record_mode, int n_blocks,
+                                  size_t lds, hipStream_t s)
 {
     switch (a.win_h) {
-    case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 18: hipLaunchKernelGGL((k_tiles<MODE, 18>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 34: hipLaunchKernelGGL((k_tiles<MODE, 34>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 66: hipLaunchKernelGGL((k_tiles<MODE, 66>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 18: hipLaunchKernelGGL((k_tiles<MODE, 18>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 34: hipLaunchKernelGGL((k_tiles<MODE, 34>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 66: hipLaunchKernelGGL((k_tiles<MODE, 66>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
+hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbuf, int record_mode, int n_blocks,
                         hipStream_t s)
 {
     if (a.tx > kMaxTX || n_blocks <= 0) return hipErrorInvalidValue;
     const size_t lds = tile_lds_bytes(a, mode);
     switch (mode) {
-    case kTileG2P: return launch_tiles_wh<kTileG2P>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
-    case kTileP2G: return launch_tiles_wh<kTileP2G>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
-    case kTileFused: return launch_tiles_wh<kTileFused>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    case kTileG2P: return launch_tiles_wh<kTileG2P>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
+    case kTileP2G: return launch_tiles_wh<kTileP2G>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
+    case kTileTail: return launch_tiles_wh<kTileTail>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
+    case kTileChain: return launch_tiles_wh<kTileChain>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -343,7 +547,8 @@ int tiles_resident_per_cu(const StepArgs &a, int mode)
     switch (mode) {
     case kTileG2P: return resident_wh<kTileG2P>(a, lds);
     case kTileP2G: return resident_wh<kTileP2G>(a, lds);
-    case kTileFused: return resident_wh<kTileFused>(a, lds);
+    case kTileTail: return resident_wh<kTileTail>(a, lds);
+    case kTileChain: return resident_wh<kTileChain>(a, lds);
     default: return 0;
     }
 }

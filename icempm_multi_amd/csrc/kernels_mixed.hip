@@ -17,14 +17,122 @@
 namespace icempm {
 namespace mixed {
 
+// Parts of the grid update that the tile kernels take over (kTileFromMomenta),
+// as in the FP64 form, on the compensated FP32 grid (6 lines per buffer).
+__device__ __forceinline__ void clear_flagged(const StepArgs &a, int zb)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (kBlock / 64);
+    for (int r = wave; r < a.n_chunks; r += n_waves) {
+        const NodeChunk run = uniform_copy(a.chunks[r]);
+        const SceneDev &S = a.scenes[run.scene];
+        glb_f *g = (glb_f *)(a.grid_mem + S.grid_off[zb]);
+        const long long gp = S.gpitch;
+        int *fl = a.dirty + zb * a.dirty_stride + S.seg_off + run.seg0;
+        const int f = lane < run.nseg ? fl[lane] : 0;
+        unsigned long long mb = __ballot(f != 0);
+        const long long node0 = (long long)run.seg0 << kSegShift;
+        while (mb) {
+            const int sg = __builtin_ctzll(mb);
+            mb &= mb - 1;
+            const long long idx = node0 + ((long long)sg << kSegShift) + lane;
+#pragma unroll
+            for (int l = 0; l < 6; l++) g[l * gp + idx] = 0;
+        }
+        if (f) fl[lane] = 0;
+    }
+}
+
+// The indenter's reaction of the substep (kernels.cuh:265-287) from the nodes
+// inside the indenter circle, one workgroup per scene (see the FP64 form).
+__device__ __forceinline__ void indenter_force_nodes(const StepArgs &a, int sc, int rb)
+{
+    const SceneDev &S = a.scenes[sc];
+    const SceneConsts c = uniform_copy(S.c);
+    if (!(c.ind_rsq > 0)) return;
+    const SceneClock clk = next_clock(c, uniform_copy(a.clocks[sc]));
+    const double r = sqrt(c.ind_rsq);
+    const int gx0 = max((int)floor((clk.ind_x - r) * c.hinv) - 1, c.xoff - c.halo);
+    const int gx1 = min((int)ceil((clk.ind_x + r) * c.hinv) + 1, c.xoff - c.halo + c.ncols - 1);
+    const int gy0 = max((int)floor((clk.ind_y - r) * c.hinv) - 1, 0);
+    const int gy1 = min((int)ceil((clk.ind_y + r) * c.hinv) + 1, c.gridY - 1);
+    if (gx1 < gx0 || gy1 < gy0) return;
+    const int ny = gy1 - gy0 + 1;
+    const long long nn = (long long)(gx1 - gx0 + 1) * ny;
+    const glb_f *g = (const glb_f *)(a.grid_mem + S.grid_off[rb]);
+    const long long gp = S.gpitch;
+    double *force = a.force_mem + S.force_off;
+    for (long long k = threadIdx.x; k < nn; k += kBlock) {
+        const int gix = gx0 + (int)(k / ny), giy = gy0 + (int)(k % ny);
+        const long long idx = (long long)(gix - c.xoff + c.halo) * c.gridY + giy;
+        const float mh = g[kLineMHi * gp + idx];
+        if (mh == 0) continue;
+        const double mass = (double)mh + (double)g[kLineMLo * gp + idx];
+        const double px = (double)g[kLinePxHi * gp + idx] + (double)g[kLinePxLo * gp + idx];
+        const double py = (double)g[kLinePyHi * gp + idx] + (double)g[kLinePyLo * gp + idx];
+        double vx, vy;
+        fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
+    }
+}
+
+// Entry k of the deferred-point list, as in the FP64 form (G2P with node
+// velocities formed from the global momenta; P2G with compensated global
+// atomics).
+__device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int rb, int wb, int k)
+{
+    const PtRef R{(unsigned *)a.pts, a.pitch};
+    const unsigned pt = (unsigned)a.defer_pts[k];
+    const int sc = a.defer_scene[k];
+    const SceneDev &S = a.scenes[sc];
+    const SceneConsts &c = S.c;
+    const ConstsF &cf = c.f;
+    const SceneClock clk = next_clock(c, a.clocks[sc]);
+    const bool record = (clk.step % c.update_every) == 0;
+    unsigned ud = R.u(kLineUtility, pt), xq, yq;
+    float D[4], Jm1, v[2], C[4], pp, qq;
+    load_state(R, pt, xq, yq, D, Jm1);
+    unsigned e;
+    StressTerms terms;
+    terms.valid = false;
+    const glb_f *gr = (const glb_f *)(a.grid_mem + S.grid_off[rb]);
+    if (!g2p_point(c, cf, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, false, xq, yq, v, C, D, Jm1, ud, pp, qq, e, pt,
+                   (unsigned)clk.step, terms)) {
+        atomicMax(&a.err[sc], e);
+        return;
+    }
+    if (!p2g) {
+        store_state(R, pt, xq, yq, v, D, C, Jm1, ud, record, pp, qq);
+        return;
+    }
+    store_state_fused(R, pt, xq, yq, D, Jm1, ud, record, pp, qq);
+    glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wb]);
+    const long long gp = S.gpitch;
+    const int gridY = c.gridY;
+    P2GStencil st;
+    const unsigned e2 = p2g_stencil(c, cf, xq, yq, v, C, D, Jm1, ud, terms, st);
+    if (e2) {
+        atomicMax(&a.err[sc], e2);
+        return;
+    }
+    st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
+        const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
+        twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
+        twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
+        twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy);
+        mark_dirty(a, S, wb, gi);
+    });
+}
+
 // 4 waves per SIMD (<= 128 VGPRs).  WHC: the window height when fixed at
-// compile time (0: read from the arguments), as in the FP64 form.
+// compile time (0: read from the arguments), as in the FP64 form; so is the
+// folded grid update of kTileFromMomenta.
 template <int MODE, int WHC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
+k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
 {
     constexpr bool DO_G2P = (MODE & kTileG2P) != 0;
     constexpr bool DO_P2G = (MODE & kTileP2G) != 0;
+    constexpr bool FROM_M = (MODE & kTileFromMomenta) != 0;
     extern __shared__ double lds_raw[];
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
@@ -40,6 +148,15 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(unsigned *)a.pts, a.pitch};
+
+    if (FROM_M) {
+        clear_flagged(a, zbuf);
+        for (int sc = blockIdx.x; sc < a.nscenes; sc += gridDim.x) indenter_force_nodes(a, sc, rbuf);
+        // the advanced clocks go to the other slot, which no block reads here
+        if (blockIdx.x == 0)
+            for (int sc = tid; sc < a.nscenes; sc += kBlock)
+                a.clocks_next[sc] = next_clock(a.scenes[sc].c, a.clocks[sc]);
+    }
 
     // Tile distribution.  Static: block b takes tiles b', b'+G, ... (b' the
     // XCD-contiguous remap of b).  Dynamic: each XCD (blockIdx.x % 8, the
@@ -84,8 +201,12 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
         glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
         int *const dflag = a.dirty + wbuf * a.dirty_stride + uniform_copy(S.seg_off);   // segment flags of wbuf
+        // the substep's clock: advanced here from the current slot, or already
+        // advanced by the grid update
+        SceneClock clk = uniform_copy(a.clocks[ti.scene]);
+        if (FROM_M) clk = next_clock(c, clk);
         bool record = false;
-        const int step = a.clocks[ti.scene].step;
+        const int step = clk.step;
         if (DO_G2P) record = record_mode >= 0 ? record_mode != 0 : (step % c.update_every) == 0;
 
         if (tid < a.tx) {
@@ -99,8 +220,44 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             run_start[tid] = s0;
             run_pre[tid + 1] = (int)(s1 - s0);
         }
+        if (DO_G2P && FROM_M) {
+            // velocities formed from the compensated momenta, one window node
+            // per thread and pass (two at once spill a register in the point
+            // loop's kernel)
+            constexpr int kU = 1;
+            for (int i0 = tid; i0 < WN; i0 += kU * kBlock) {
+                float g6[kU][6];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    const int i = i0 + u * kBlock;
+                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    const bool in = i < WN && col >= 0 && col < c.ncols && row >= 0 && row < gridY;
+                    const long long gi = (long long)col * gridY + row;
+#pragma unroll
+                    for (int l = 0; l < 6; l++) g6[u][l] = in ? (float)gr[l * gp + gi] : 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    const int i = i0 + u * kBlock;
+                    if (i >= WN) break;
+                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    float vx = 0, vy = 0;
+                    if (g6[u][kLineMHi] != 0) {
+                        const double mass = (double)g6[u][kLineMHi] + (double)g6[u][kLineMLo];
+                        const double px = (double)g6[u][kLinePxHi] + (double)g6[u][kLinePxLo];
+                        const double py = (double)g6[u][kLinePyHi] + (double)g6[u][kLinePyLo];
+                        double dvx, dvy;
+                        fp64::node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, mass, px, py, dvx, dvy);
+                        vx = (float)dvx;
+                        vy = (float)dvy;
+                    }
+                    vw[2 * i] = vx;
+                    vw[2 * i + 1] = vy;
+                }
+            }
+        }
         for (int i = tid; i < WN; i += kBlock) {
-            if (DO_G2P) {
+            if (DO_G2P && !FROM_M) {
                 const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                 float vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
@@ -183,9 +340,17 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
             if (DO_G2P) {
                 float pp, qq;
                 unsigned e;
-                if (!g2p_point(c, cf, gr, gp, vw, wc0, wr0, win_w, win_h, xq, yq, v, C, D, Jm1, ud, pp, qq,
-                               e, (unsigned)pt, (unsigned)step, terms)) {
-                    err = max(err, e);
+                // a point whose G2P stencil left the window goes to the
+                // deferred list, which k_deferred handles after this launch
+                if (!g2p_point(c, cf, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, xq, yq, v, C, D, Jm1, ud,
+                               pp, qq, e, (unsigned)pt, (unsigned)step, terms)) {
+                    if (e) {
+                        err = max(err, e);
+                    } else {
+                        const int slot = atomicAdd(a.defer_count, 1);
+                        a.defer_pts[slot] = (int)pt;
+                        a.defer_scene[slot] = ti.scene;
+                    }
                     continue;
                 }
                 if (DO_P2G) store_state_fused(R, pt, xq, yq, D, Jm1, ud, record, pp, qq);
@@ -254,6 +419,35 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int record_mode)
     }
 }
 
+// ---------------------------------------------------------------------------
+// The points a kTileFromMomenta launch set aside (their G2P stencil left the
+// tile's LDS window), handled by the launch that follows it: the point list
+// needs every workgroup of that launch to have finished.  A few workgroups
+// suffice (points drift out of their window rarely); the last to finish
+// empties the list.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_deferred(StepArgs a, int p2g, int rbuf, int wbuf)
+{
+    const int n = *a.defer_count;
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        deferred_point(a, p2g != 0, rbuf, wbuf, k);
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(a.defer_count + 1, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        a.defer_count[0] = 0;
+        a.defer_count[1] = 0;
+    }
+}
+
+hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s)
+{
+    if (n_blocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_deferred, dim3(n_blocks), dim3(kBlock), 0, s, a, (mode & kTileP2G) ? 1 : 0, rbuf, wbuf);
+    return hipGetLastError();
+}
+
 size_t tile_lds_bytes(const StepArgs &a, int mode)
 {
     const size_t WN = (size_t)a.win_w * a.win_h;
@@ -265,30 +459,31 @@ size_t tile_lds_bytes(const StepArgs &a, int mode)
 
 // window heights compiled in: ty = 16, 32, 64 with a drift margin of 1 or 0
 template <int MODE>
-static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int record_mode, int n_blocks, size_t lds,
-                                  hipStream_t s)
+static hipError_t launch_tiles_wh(const StepArgs &a, int rbuf, int wbuf, int zbuf, int record_mode, int n_blocks,
+                                  size_t lds, hipStream_t s)
 {
     switch (a.win_h) {
-    case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 18: hipLaunchKernelGGL((k_tiles<MODE, 18>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 34: hipLaunchKernelGGL((k_tiles<MODE, 34>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    case 66: hipLaunchKernelGGL((k_tiles<MODE, 66>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
-    default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, record_mode); break;
+    case 20: hipLaunchKernelGGL((k_tiles<MODE, 20>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 36: hipLaunchKernelGGL((k_tiles<MODE, 36>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 68: hipLaunchKernelGGL((k_tiles<MODE, 68>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 18: hipLaunchKernelGGL((k_tiles<MODE, 18>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 34: hipLaunchKernelGGL((k_tiles<MODE, 34>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    case 66: hipLaunchKernelGGL((k_tiles<MODE, 66>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
+    default: hipLaunchKernelGGL((k_tiles<MODE, 0>), dim3(n_blocks), dim3(kBlock), lds, s, a, rbuf, wbuf, zbuf, record_mode); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int record_mode, int n_blocks,
+hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbuf, int record_mode, int n_blocks,
                         hipStream_t s)
 {
     if (a.tx > kMaxTX || n_blocks <= 0) return hipErrorInvalidValue;
     const size_t lds = tile_lds_bytes(a, mode);
     switch (mode) {
-    case kTileG2P: return launch_tiles_wh<kTileG2P>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
-    case kTileP2G: return launch_tiles_wh<kTileP2G>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
-    case kTileFused: return launch_tiles_wh<kTileFused>(a, rbuf, wbuf, record_mode, n_blocks, lds, s);
+    case kTileG2P: return launch_tiles_wh<kTileG2P>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
+    case kTileP2G: return launch_tiles_wh<kTileP2G>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
+    case kTileTail: return launch_tiles_wh<kTileTail>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
+    case kTileChain: return launch_tiles_wh<kTileChain>(a, rbuf, wbuf, zbuf, record_mode, n_blocks, lds, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -316,7 +511,8 @@ int tiles_resident_per_cu(const StepArgs &a, int mode)
     switch (mode) {
     case kTileG2P: return resident_wh<kTileG2P>(a, lds);
     case kTileP2G: return resident_wh<kTileP2G>(a, lds);
-    case kTileFused: return resident_wh<kTileFused>(a, lds);
+    case kTileTail: return resident_wh<kTileTail>(a, lds);
+    case kTileChain: return resident_wh<kTileChain>(a, lds);
     default: return 0;
     }
 }

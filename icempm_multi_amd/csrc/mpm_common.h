@@ -74,7 +74,7 @@ struct SceneDev {
     long long pt_off;       // first point of the scene in the SoA
     int n_pts;              // points of the scene (after the last compaction)
     int cap_pts;            // capacity of the scene's point range
-    long long grid_off[2];  // byte offsets of the two accumulation grids from StepArgs::grid_mem
+    long long grid_off[3];  // byte offsets of the three accumulation grids from StepArgs::grid_mem
                             // (3 lines of gpitch elements, or 6 for mixed)
     long long gpitch;       // elements per grid line (>= ncols*gridY)
     long long cell_off;     // first entry of the scene in the cell_start array
@@ -132,8 +132,14 @@ struct StepArgs {
     int lane_spread;          // lane-to-point map of the tile kernels: 0 identity, 1 over 16 cells, 2 over 32 cells
     int dyn_tiles;            // 1: tiles pulled from per-XCD atomic queues, 0: static round robin
     int *tile_queue;          // 8 queue counters + 1 finished-block counter, zero between launches
-    int *dirty;               // segment flags: [buffer 0 of all scenes][buffer 1 of all scenes]
+    int *dirty;               // segment flags: [buffer 0 of all scenes][buffer 1 ...][buffer 2 ...]
     long long dirty_stride;   // segments per buffer (all scenes)
+    // Points a kTileFromMomenta launch sets aside for k_deferred (one slot per
+    // point of the point space; defer_count[0] = entries, [1] = k_deferred's
+    // finished workgroups; both zero between launches).
+    int *defer_pts;
+    int *defer_scene;
+    int *defer_count;
 };
 
 // The clock of the next substep (model.cpp:39-41): t += dt by repeated

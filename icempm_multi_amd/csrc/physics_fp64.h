@@ -111,7 +111,8 @@ __device__ __forceinline__ unsigned stencil_error(const SceneConsts &c, int bxc,
 
 // Velocity of one grid node with mass (kernels.cuh:178-300, live lines): p/m,
 // gravity, the vmax clamp, the moving indenter (its reaction force binned by
-// angle into `force`), and the walls.  gix, giy are global node indices.
+// angle into `force`, unless force is null), and the walls.  gix, giy are
+// global node indices.
 __device__ __forceinline__ void node_velocity(const SceneConsts &c, const SceneClock &clk, double *force, int gix,
                                               int giy, double mass, double px, double py, double &vx_out,
                                               double &vy_out)
@@ -136,13 +137,15 @@ __device__ __forceinline__ void node_velocity(const SceneConsts &c, const SceneC
             const double pv0 = vx, pv1 = vy;
             vx = c.ind_vel + vt0;
             vy = 0 + vt1;
-            const double f0 = (pv0 - vx) * mass / c.dt, f1 = (pv1 - vy) * mass / c.dt;
-            float angle = atan2f((float)n0, (float)n1);
-            angle += kPi;
-            angle *= c.n_ind_sub / (2 * kPi);
-            const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
-            atomicAdd(&force[2 * index], f0);
-            atomicAdd(&force[2 * index + 1], f1);
+            if (force) {
+                const double f0 = (pv0 - vx) * mass / c.dt, f1 = (pv1 - vy) * mass / c.dt;
+                float angle = atan2f((float)n0, (float)n1);
+                angle += kPi;
+                angle *= c.n_ind_sub / (2 * kPi);
+                const int index = min(max((int)angle, 0), c.n_ind_sub - 1);
+                atomicAdd(&force[2 * index], f0);
+                atomicAdd(&force[2 * index + 1], f1);
+            }
         }
     }
     if (giy <= 2) { vx = 0; vy = 0; }

@@ -53,14 +53,34 @@ __device__ __forceinline__ int xcd_block()
     return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
 }
 
+// Velocity of a node from the momenta of grid g (a G2P gather outside the LDS
+// window).
+__device__ __forceinline__ void node_velocity_global(const SceneConsts &c, const SceneClock &clk,
+                                                               const glb_d *g, long long gpitch, int col, int row,
+                                                               double &vx, double &vy)
+{
+    const long long gi = (long long)col * c.gridY + row;
+    vx = 0;
+    vy = 0;
+    const double m = g[gi];
+    if (m != 0) node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, m, g[gpitch + gi], g[2 * gpitch + gi], vx, vy);
+}
+
 // G2P of one point: gathers v and C (kernels.cuh:344-372), advects x, updates F
 // (375-376) and runs the constitutive branch (378-391).  Returns false (and
 // sets the error byte) if the stencil leaves the grid.
+//
+// vw: the LDS window of node velocities (null: every node from global memory).
+// vclk: null if grid g holds node velocities (a grid update ran); otherwise g
+// holds momenta and a node read from global memory gets its velocity formed
+// here with that clock (node_velocity, no force).
+// defer_outside: if the stencil leaves the window, return false with e = 0
+// and change nothing (the caller handles the point later).
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, long long gpitch,
-                                          const lds_d *vw, int wc0, int wr0, int win_w,
-                                          int win_h, double &x, double &y, double v[2], double B[4], double F[4],
-                                          double &Jp, long long &ud, double &p_out, double &q_out, unsigned &e,
-                                          StressTerms &st)
+                                          const SceneClock *vclk, const lds_d *vw, int wc0, int wr0, int win_w,
+                                          int win_h, bool defer_outside, double &x, double &y, double v[2],
+                                          double B[4], double F[4], double &Jp, long long &ud, double &p_out,
+                                          double &q_out, unsigned &e, StressTerms &st)
 {
     int bx, by;
     double fx, fy;
@@ -74,6 +94,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     weights(fy, wy);
     const int lx = bxc - wc0, ly = by - wr0;
     const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
+    if (!inwin && defer_outside) return false;   // e = 0
     // The weight of node (i, j) is wx[i] * wy[j], so the sums separate: per
     // column i, a = sum_j wy[j] v_ij and b = sum_j wy[j] (j - fy) v_ij, then
     // v = sum_i wx[i] a, C = (4/h) sum_i (wx[i] (i - fx) a, wx[i] b)
@@ -102,6 +123,22 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
             for (int j = 0; j < 3; j++) {
                 const int li = (lx + i) * win_h + ly + j;
                 const double nvx = vw[2 * li], nvy = vw[2 * li + 1];   // one 128-bit LDS read
+                ax += wy[j] * nvx;
+                ay += wy[j] * nvy;
+                bx += wyd[j] * nvx;
+                by_ += wyd[j] * nvy;
+            }
+            column(i, ax, ay, bx, by_);
+        }
+    } else if (vclk) {
+        // momenta in global memory: one node at a time, its velocity formed here
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) {
+            double ax = 0, ay = 0, bx = 0, by_ = 0;
+#pragma unroll 1
+            for (int j = 0; j < 3; j++) {
+                double nvx, nvy;
+                node_velocity_global(c, *vclk, g, gpitch, bxc + i, by + j, nvx, nvy);
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;
                 bx += wyd[j] * nvx;

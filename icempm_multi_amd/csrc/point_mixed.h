@@ -129,13 +129,35 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
     }
 }
 
-// G2P of one point from an LDS window of node velocities (or the global
-// velocity lines if the stencil left the window).
+// Velocity of a node from the compensated momenta of grid g (a G2P gather
+// outside the LDS window), formed as the grid update forms it.
+__device__ __forceinline__ void node_velocity_global(const SceneConsts &c, const SceneClock &clk, const glb_f *g,
+                                                     long long gp, int col, int row, float &vx, float &vy)
+{
+    const long long gi = (long long)col * c.gridY + row;
+    vx = 0;
+    vy = 0;
+    const float mh = g[kLineMHi * gp + gi];
+    if (mh == 0) return;
+    const double mass = (double)mh + (double)g[kLineMLo * gp + gi];
+    const double px = (double)g[kLinePxHi * gp + gi] + (double)g[kLinePxLo * gp + gi];
+    const double py = (double)g[kLinePyHi * gp + gi] + (double)g[kLinePyLo * gp + gi];
+    double dvx, dvy;
+    fp64::node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, mass, px, py, dvx, dvy);
+    vx = (float)dvx;
+    vy = (float)dvy;
+}
+
+// G2P of one point from an LDS window of node velocities, or from global
+// memory if the stencil left the window: the velocity lines, or (vclk not
+// null) node velocities formed from the momenta with that clock.
+// defer_outside: if the stencil leaves the window, return false with e = 0
+// and change nothing (the caller handles the point later).
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_f *g, long long gp,
-                                          const lds_f *vw, int wc0, int wr0, int win_w,
-                                          int win_h, unsigned &xq, unsigned &yq, float v[2], float C[4], float D[4],
-                                          float &Jm1, unsigned &ud, float &p_out, float &q_out, unsigned &e,
-                                          unsigned point, unsigned step, StressTerms &st)
+                                          const SceneClock *vclk, const lds_f *vw, int wc0, int wr0, int win_w,
+                                          int win_h, bool defer_outside, unsigned &xq, unsigned &yq, float v[2],
+                                          float C[4], float D[4], float &Jm1, unsigned &ud, float &p_out,
+                                          float &q_out, unsigned &e, unsigned point, unsigned step, StressTerms &st)
 {
     int bx, by;
     float fx, fy;
@@ -149,6 +171,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
     weights(fy, wy);
     const int lx = bxc - wc0, ly = by - wr0;
     const bool inwin = vw && lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h;
+    if (!inwin && defer_outside) return false;   // e = 0
     // factored sums, as in the FP64 form: per column i, a = sum_j wy[j] v_ij
     // and b = sum_j wy[j] (j - fy) v_ij; v = sum_i wx[i] a and
     // C = (4/h) sum_i (wx[i] (i - fx) a, wx[i] b)
@@ -174,6 +197,22 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
             for (int j = 0; j < 3; j++) {
                 const int li = (lx + i) * win_h + ly + j;
                 const float nvx = vw[2 * li], nvy = vw[2 * li + 1];   // (vx, vy) interleaved: one 64-bit LDS read
+                ax += wy[j] * nvx;
+                ay += wy[j] * nvy;
+                bx_ += wyd[j] * nvx;
+                by_ += wyd[j] * nvy;
+            }
+            column(i, ax, ay, bx_, by_);
+        }
+    } else if (vclk) {
+        // momenta in global memory: one node at a time, its velocity formed here
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) {
+            float ax = 0, ay = 0, bx_ = 0, by_ = 0;
+#pragma unroll 1
+            for (int j = 0; j < 3; j++) {
+                float nvx, nvy;
+                node_velocity_global(c, *vclk, g, gp, bxc + i, by + j, nvx, nvy);
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;
                 bx_ += wyd[j] * nvx;

@@ -113,7 +113,7 @@ def check_scene(b, i, P, n, mid, final, precision, rng):
     assert b.scene_points(i) == n
     assert np.all(np.isfinite(final[1:16, :n])), f"scene {i}: non-finite state"
     nn = O.n_nodes(P)
-    mass = b.get_grid(i, 0, nn)[0].sum() + b.get_grid(i, 1, nn)[0].sum()
+    mass = sum(b.get_grid(i, k, nn)[0].sum() for k in range(3))   # one buffer holds the last P2G
     m = P.ParticleMass if precision == im.PRECISION_FP64 else float(np.float32(P.ParticleMass))
     assert abs(mass - n * m) <= 1e-9 * n * m, f"scene {i}: grid mass {mass} vs {n * m}"
     keys, perm, counts = b.get_bins(i, n, nn)

@@ -67,7 +67,7 @@ def test_partition_call_order_and_sizes(precision):
     refused(im.ERR_ARGUMENT, part.set_lane_map, 3)
     refused(im.ERR_ARGUMENT, part.set_tile_schedule, 2)
     nodes = (P.GridXTotal + 2 * P.GridHaloSize) * P.GridY
-    refused(im.ERR_ARGUMENT, part.get_grid, 2, nodes)
+    refused(im.ERR_ARGUMENT, part.get_grid, 3, nodes)      # buffers 0, 1, 2
     refused(im.ERR_ARGUMENT, part.get_grid, 0, nodes + 1)
     part.sort_points()
     steps_like_oracle(part, P, h, n, 1e-9 if precision == im.PRECISION_FP64 else 1e-6)
