@@ -93,6 +93,8 @@ _SIGS = {
     "icempm_poisson_disk": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, _D, _D,
                                       C.c_longlong, C.POINTER(C.c_longlong)]),
     "icempm_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    "icempm_substeps_noalloc": (C.c_int, [C.c_int, _P, C.c_size_t, C.c_void_p, C.c_uint, C.c_int, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]),
     "icempm_batch_get_tile_config": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                C.POINTER(C.c_int)]),
     "icempm_batch_get_graph_stats": (C.c_int, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),

@@ -330,6 +330,29 @@ int icempm_batch_reset_kernel_stats(icempm_batch *b);
 int icempm_poisson_disk(double r, double x0, double y0, double x1, double y1, uint64_t seed, double *x, double *y,
                         long long cap, long long *n);
 
+/* ---- allocation-free substeps -------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc): n_substeps substeps of one scene on points the caller holds in
+ * device memory, FP64, in the HostSideSOA layout (16 lines of `pitch`
+ * doubles, parameters_sim.h:27-35), with the unbinned P2G and G2P of
+ * icempm_partition_step(..., ICEMPM_STEP_DIRECT_P2G).  The clock starts at
+ * prms->SimulationTime / SimulationStep and the indenter at
+ * prms->indenter_x / indenter_y.  Every other buffer (grid, node flags,
+ * clocks, error byte, force bins) is a static device array of the library,
+ * about 24 MiB per GPU, held until the process ends; grids of more than
+ * ICEMPM_NOALLOC_MAX_NODES nodes (halo columns included) are refused with
+ * ICEMPM_ERR_CAPACITY.  The call allocates nothing and is asynchronous on
+ * `stream` (a hipStream_t of `device`, or NULL for its null stream).  If
+ * non-null, the device error byte goes to *d_err_out and the indenter force
+ * bins (2 * n_indenter_subdivisions doubles, summed over the call's
+ * substeps) to d_force_out, both device memory.  Calls that overlap on one
+ * GPU share the static array and race. */
+#define ICEMPM_NOALLOC_MAX_NODES (1 << 20)
+int icempm_substeps_noalloc(int device, const icempm_simparams *prms, size_t prms_bytes, double *d_points,
+                            unsigned pitch, int n_points, int n_substeps, void *stream, unsigned *d_err_out,
+                            double *d_force_out);
+
 #ifdef __cplusplus
 }
 #endif
