@@ -234,6 +234,7 @@ def batch_summary(args, im, scenes, sheet_fp64_value):
     log(f"batch: {len(bag)} scenes, {n_tot} points, setup {time.perf_counter() - t0:.1f}s")
     K = args.batch_steps
     frame = min(P.UpdateEveryNthStep for P, _, _ in bag)
+    G = im.STEP_GRAPH if args.mode == "graph" else 0       # --mode eager: direct launches
 
     # plain: one partition per scene, stepped one after the other
     parts = []
@@ -249,12 +250,12 @@ def batch_summary(args, im, scenes, sheet_fp64_value):
         for p in parts:
             p.step(m, flags)
 
-    prime(lambda: frames(plain_step, K, frame, im.STEP_GRAPH, im.STEP_RESORT),
+    prime(lambda: frames(plain_step, K, frame, G, im.STEP_RESORT),
           lambda: sum(p.graph_stats()[0] for p in parts))
     for p in parts:
         p.synchronize()
     t0 = time.perf_counter()
-    frames(plain_step, K, frame, im.STEP_GRAPH, im.STEP_RESORT)
+    frames(plain_step, K, frame, G, im.STEP_RESORT)
     for p in parts:
         p.synchronize()
     plain = n_tot * K / (time.perf_counter() - t0) / 1e6
@@ -272,9 +273,9 @@ def batch_summary(args, im, scenes, sheet_fp64_value):
         chosen, _ = bt.autotune(2)
         tile = bt.tile_config()
         log(f"[batch {name}] autotune: candidate {chosen} {tile}")
-        prime(lambda: frames(bt.step, K, frame, im.STEP_GRAPH, im.STEP_RESORT), lambda: bt.graph_stats()[0])
+        prime(lambda: frames(bt.step, K, frame, G, im.STEP_RESORT), lambda: bt.graph_stats()[0])
         t0 = time.perf_counter()
-        frames(bt.step, K, frame, im.STEP_GRAPH, im.STEP_RESORT)
+        frames(bt.step, K, frame, G, im.STEP_RESORT)
         wall = time.perf_counter() - t0
         if bt.error_code:
             raise RuntimeError(f"batch {name}: device error code {bt.error_code}")
@@ -286,7 +287,7 @@ def batch_summary(args, im, scenes, sheet_fp64_value):
             for P, h, n in bag:
                 bt.add_scene(P, h, n)
             bt.commit()
-            bt.step(args.accuracy_steps, im.STEP_GRAPH | im.STEP_RESORT)
+            bt.step(args.accuracy_steps, G | im.STEP_RESORT)
             if bt.error_code:
                 raise RuntimeError(f"batch accuracy {name}: device error code {bt.error_code}")
             outs = []
@@ -320,7 +321,8 @@ def batch_summary(args, im, scenes, sheet_fp64_value):
         "mixed_batched_vs_fp64_batched": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
         "mixed_batched_vs_fp64_plain_per_scene": round(res["mixed"]["value"] / plain, 3),
         "mixed_batched_vs_fp64_sheet": round(res["mixed"]["value"] / sheet_fp64_value, 3),
-        "plain_mode": "one partition per scene, scenes in turn, each replaying its own hipGraph chunks",
+        "plain_mode": ("one partition per scene, scenes in turn, each replaying its own hipGraph chunks"
+                       if G else "one partition per scene, scenes in turn, direct launches"),
         "tile": {k: v["tile"] for k, v in res.items()},
         "accuracy_mixed_vs_fp64": acc,
     }
@@ -336,6 +338,7 @@ def floes_workload(args, im, scenes):
     log(f"floes: {len(bag)} scenes, {n_tot} points, setup {time.perf_counter() - t0:.1f}s")
     K = args.steps
     frame = min(P.UpdateEveryNthStep for P, _, _ in bag)
+    G = im.STEP_GRAPH if args.mode == "graph" else 0       # --mode eager: direct launches
     parts = []
     for P, h, n in bag:
         p = im.Partition(0, 0, im.PRECISION_FP64)
@@ -349,10 +352,10 @@ def floes_workload(args, im, scenes):
         for p in parts:
             p.step(m, flags)
 
-    prime(lambda: frames(plain_step, K, frame, im.STEP_GRAPH, im.STEP_RESORT),
+    prime(lambda: frames(plain_step, K, frame, G, im.STEP_RESORT),
           lambda: sum(p.graph_stats()[0] for p in parts))
     t0 = time.perf_counter()
-    frames(plain_step, K, frame, im.STEP_GRAPH, im.STEP_RESORT)
+    frames(plain_step, K, frame, G, im.STEP_RESORT)
     for p in parts:
         p.synchronize()
     plain = n_tot * K / (time.perf_counter() - t0) / 1e6
@@ -366,9 +369,9 @@ def floes_workload(args, im, scenes):
         bt.commit()
         bt.autotune(2)
         tile = bt.tile_config()
-        prime(lambda: frames(bt.step, K, frame, im.STEP_GRAPH, im.STEP_RESORT), lambda: bt.graph_stats()[0])
+        prime(lambda: frames(bt.step, K, frame, G, im.STEP_RESORT), lambda: bt.graph_stats()[0])
         t0 = time.perf_counter()
-        frames(bt.step, K, frame, im.STEP_GRAPH, im.STEP_RESORT)
+        frames(bt.step, K, frame, G, im.STEP_RESORT)
         wall = time.perf_counter() - t0
         if bt.error_code:
             raise RuntimeError(f"floes {name}: device error code {bt.error_code}")
@@ -380,8 +383,8 @@ def floes_workload(args, im, scenes):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"{len(bag)} floe fields, {n_tot} points, convex floes 1-20 m (power law -2), "
-                                "grids 4096x1024..8192x2048, h=0.05 m, dt=1e-5 s, mixed form, batched, hipGraph "
-                                "replay"), "scenes": len(bag), "points": n_tot},
+                                "grids 4096x1024..8192x2048, h=0.05 m, dt=1e-5 s, mixed form, batched, "
+                                + ("hipGraph replay" if G else "direct launches")), "scenes": len(bag), "points": n_tot},
         "fp64_batched": round(res["fp64"]["value"], 2),
         "fp64_plain_per_scene": round(plain, 2),
         "mixed_batched_vs_fp64_batched": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
@@ -527,7 +530,8 @@ def main():
         "roofline": primary["roofline"],
         "kernel_ms": primary["kernel_ms"],
         "timing_pass": f"kernel_ms and roofline.avg_launch_us: a {args.timing_substeps}-substep call after the timed "
-                       "region with hipEvents around every launch (recorded inside the replayed graphs)",
+                       "region with hipEvents around every launch" +
+                       (" (recorded inside the replayed graphs)" if args.mode == "graph" else ""),
     }
     if args.precision == "fp64" and not args.no_mixed:
         mx = timed("mixed")

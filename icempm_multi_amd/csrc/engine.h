@@ -201,7 +201,12 @@ private:
     };
     typedef std::array<int, 7> GraphKey;
     static constexpr int kMaxChunk = 64;      // units per graph: at most 192 kernel nodes (3 per unfused unit)
-    static constexpr size_t kMaxGraphs = 256; // the cache is cleared beyond this
+    static constexpr size_t kMaxGraphs = 256; // the cache is cleared between calls beyond this
+    // Graphs one call can add: head and tail 1 each; the chain's 64-unit
+    // chunks start from each of the 3 grid buffers, and its shorter chunks
+    // (32, 16, ..., 1 units) occur once each (9 + 2 = 11; unfused and
+    // unbinned runs: 7).
+    static constexpr size_t kMaxGraphsPerCall = 16;
     std::map<GraphKey, GraphExec> graphs_;
     int pts_flip_ = 0;                        // which buffer of the pair pts_[0] is (the sort swaps them)
     long long n_captures_ = 0;                // graphs captured since creation

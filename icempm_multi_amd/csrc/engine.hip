@@ -1075,7 +1075,10 @@ int Engine::chunk_graph(int kind, int n, int b, int cs, int tmode, bool first_ti
     const GraphKey key{kind, n, b, cs, pts_flip_, tmode, first_timed ? 1 : 0};
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
-        if (graphs_.size() >= kMaxGraphs) invalidate_graphs();
+        // step() makes room before a call's first launch: clearing the cache
+        // here would destroy graphs this call launched and still reads
+        if (graphs_.size() >= kMaxGraphs + kMaxGraphsPerCall)
+            return fail(ICEMPM_ERR_STATE, "graph cache over its bound within a call");
         // The capture records the timing events into the graph: it takes them
         // from the pool (emptied first) and keeps them.
         if (tmode) RC(collect_timings());
@@ -1235,6 +1238,10 @@ int Engine::step(int nsub, int flags)
     const bool use_graph = (flags & ICEMPM_STEP_GRAPH) != 0;
     const int tmode = timed ? ((flags & ICEMPM_STEP_SAMPLED) ? 2 : 1) : 0;
 
+    // A call captures at most kMaxGraphsPerCall chunk graphs; the cache is
+    // cleared between calls (the last call's replays have completed: it
+    // synchronised on ev_b_), never within one.
+    if (use_graph && graphs_.size() + kMaxGraphsPerCall > kMaxGraphs) invalidate_graphs();
     int start_buf = 0;
     RC(start_buffer(fused, direct, start_buf));
     HIPCHK(hipEventRecord(ev_a_, stream_));

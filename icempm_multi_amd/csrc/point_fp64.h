@@ -53,19 +53,6 @@ __device__ __forceinline__ int xcd_block()
     return (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
 }
 
-// Velocity of a node from the momenta of grid g (a G2P gather outside the LDS
-// window).
-__device__ __forceinline__ void node_velocity_global(const SceneConsts &c, const SceneClock &clk,
-                                                               const glb_d *g, long long gpitch, int col, int row,
-                                                               double &vx, double &vy)
-{
-    const long long gi = (long long)col * c.gridY + row;
-    vx = 0;
-    vy = 0;
-    const double m = g[gi];
-    if (m != 0) node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, m, g[gpitch + gi], g[2 * gpitch + gi], vx, vy);
-}
-
 // G2P of one point: gathers v and C (kernels.cuh:344-372), advects x, updates F
 // (375-376) and runs the constitutive branch (378-391).  Returns false (and
 // sets the error byte) if the stencil leaves the grid.
@@ -131,14 +118,27 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
             column(i, ax, ay, bx, by_);
         }
     } else if (vclk) {
-        // momenta in global memory: one node at a time, its velocity formed here
-#pragma unroll 1
+        // momenta in global memory (the deferred-point kernel, which has
+        // registers to spare): the 27 loads of the 9 nodes issued together,
+        // then each node's velocity formed here
+        double gm[9], gpx[9], gpy[9];
+#pragma unroll
+        for (int n = 0; n < 9; n++) {
+            const long long gi = (long long)(bxc + n / 3) * c.gridY + by + n % 3;
+            gm[n] = g[gi];
+            gpx[n] = g[gpitch + gi];
+            gpy[n] = g[2 * gpitch + gi];
+        }
+#pragma unroll
         for (int i = 0; i < 3; i++) {
             double ax = 0, ay = 0, bx = 0, by_ = 0;
-#pragma unroll 1
+#pragma unroll
             for (int j = 0; j < 3; j++) {
-                double nvx, nvy;
-                node_velocity_global(c, *vclk, g, gpitch, bxc + i, by + j, nvx, nvy);
+                const int n = 3 * i + j;
+                double nvx = 0, nvy = 0;
+                if (gm[n] != 0)
+                    node_velocity(c, *vclk, nullptr, bxc + i - c.halo + c.xoff, by + j, gm[n], gpx[n], gpy[n], nvx,
+                                  nvy);
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;
                 bx += wyd[j] * nvx;

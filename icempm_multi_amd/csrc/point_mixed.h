@@ -129,25 +129,6 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
     }
 }
 
-// Velocity of a node from the compensated momenta of grid g (a G2P gather
-// outside the LDS window), formed as the grid update forms it.
-__device__ __forceinline__ void node_velocity_global(const SceneConsts &c, const SceneClock &clk, const glb_f *g,
-                                                     long long gp, int col, int row, float &vx, float &vy)
-{
-    const long long gi = (long long)col * c.gridY + row;
-    vx = 0;
-    vy = 0;
-    const float mh = g[kLineMHi * gp + gi];
-    if (mh == 0) return;
-    const double mass = (double)mh + (double)g[kLineMLo * gp + gi];
-    const double px = (double)g[kLinePxHi * gp + gi] + (double)g[kLinePxLo * gp + gi];
-    const double py = (double)g[kLinePyHi * gp + gi] + (double)g[kLinePyLo * gp + gi];
-    double dvx, dvy;
-    fp64::node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, mass, px, py, dvx, dvy);
-    vx = (float)dvx;
-    vy = (float)dvy;
-}
-
 // G2P of one point from an LDS window of node velocities, or from global
 // memory if the stencil left the window: the velocity lines, or (vclk not
 // null) node velocities formed from the momenta with that clock.
@@ -205,14 +186,32 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
             column(i, ax, ay, bx_, by_);
         }
     } else if (vclk) {
-        // momenta in global memory: one node at a time, its velocity formed here
-#pragma unroll 1
+        // compensated momenta in global memory (the deferred-point kernel,
+        // which has registers to spare): the loads of the 9 nodes issued
+        // together, then each node's velocity formed here
+        float gl[9][6];
+#pragma unroll
+        for (int n = 0; n < 9; n++) {
+            const long long gi = (long long)(bxc + n / 3) * c.gridY + by + n % 3;
+#pragma unroll
+            for (int l = 0; l < 6; l++) gl[n][l] = g[l * gp + gi];
+        }
+#pragma unroll
         for (int i = 0; i < 3; i++) {
             float ax = 0, ay = 0, bx_ = 0, by_ = 0;
-#pragma unroll 1
+#pragma unroll
             for (int j = 0; j < 3; j++) {
-                float nvx, nvy;
-                node_velocity_global(c, *vclk, g, gp, bxc + i, by + j, nvx, nvy);
+                const float *q = gl[3 * i + j];
+                float nvx = 0, nvy = 0;
+                if (q[kLineMHi] != 0) {
+                    double dvx, dvy;
+                    fp64::node_velocity(c, *vclk, nullptr, bxc + i - c.halo + c.xoff, by + j,
+                                        (double)q[kLineMHi] + (double)q[kLineMLo],
+                                        (double)q[kLinePxHi] + (double)q[kLinePxLo],
+                                        (double)q[kLinePyHi] + (double)q[kLinePyLo], dvx, dvy);
+                    nvx = (float)dvx;
+                    nvy = (float)dvy;
+                }
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;
                 bx_ += wyd[j] * nvx;
