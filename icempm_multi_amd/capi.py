@@ -93,6 +93,7 @@ _SIGS = {
     "icempm_poisson_disk": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, _D, _D,
                                       C.c_longlong, C.POINTER(C.c_longlong)]),
     "icempm_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    "icempm_tile_candidates": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
     "icempm_substeps_noalloc": (C.c_int, [C.c_int, _P, C.c_size_t, C.c_void_p, C.c_uint, C.c_int, C.c_int,
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "icempm_batch_get_tile_config": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -185,6 +186,15 @@ def poisson_disk(r, x0, y0, x1, y1, seed):
     return x[:n.value].copy(), y[:n.value].copy()
 
 
+def tile_candidates():
+    """The autotuner's candidate shapes [(tx, ty, lane_spread), ...]; autotune()
+    reports them at margin 0, then at margin 1, in this order."""
+    k = lib().icempm_tile_candidates(None, None, None, 0)
+    tx, ty, ls = (C.c_int * k)(), (C.c_int * k)(), (C.c_int * k)()
+    lib().icempm_tile_candidates(tx, ty, ls, k)
+    return list(zip(tx, ty, ls))
+
+
 # ---------------------------------------------------------------- partition
 
 class Partition:
@@ -269,12 +279,14 @@ class Partition:
         _check(lib().icempm_partition_set_tile_schedule(self._h, dynamic))
 
     def autotune(self, n_trials=5):
-        """Returns (chosen index, ms per fused launch of the 14 candidates at
-        margin 0 then margin 1); tile_config() gives the shape kept."""
-        out = (C.c_float * 32)()
+        """Returns (chosen index, ms per substep launch of the candidates of
+        tile_candidates() at margin 0 then margin 1); tile_config() gives the
+        shape kept."""
+        k = 2 * len(tile_candidates())
+        out = (C.c_float * k)()
         chosen = C.c_int(-1)
-        _check(lib().icempm_partition_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
-        return chosen.value, list(out)[:28]
+        _check(lib().icempm_partition_autotune(self._h, n_trials, out, k, C.byref(chosen)))
+        return chosen.value, list(out)
 
     def tile_config(self):
         """{tx, ty, margin, lane_spread} of the binned kernels."""
@@ -426,10 +438,11 @@ class Batch:
         return fb.value, ox.value, oy.value
 
     def autotune(self, n_trials=3):
-        out = (C.c_float * 32)()
+        k = 2 * len(tile_candidates())
+        out = (C.c_float * k)()
         chosen = C.c_int(-1)
-        _check(lib().icempm_batch_autotune(self._h, n_trials, out, 32, C.byref(chosen)))
-        return chosen.value, list(out)[:28]
+        _check(lib().icempm_batch_autotune(self._h, n_trials, out, k, C.byref(chosen)))
+        return chosen.value, list(out)
 
     def tile_config(self):
         v = [C.c_int() for _ in range(4)]

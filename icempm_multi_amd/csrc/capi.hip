@@ -353,6 +353,16 @@ int icempm_partition_set_tile_schedule(icempm_partition *p, int dynamic)
     });
 }
 
+int icempm_tile_candidates(int *tx, int *ty, int *lane_spread, int n)
+{
+    for (int k = 0; k < n && k < icempm::kNumTileCandidates; k++) {
+        if (tx) tx[k] = icempm::kTileCandidates[k][0];
+        if (ty) ty[k] = icempm::kTileCandidates[k][1];
+        if (lane_spread) lane_spread[k] = icempm::kTileCandidates[k][2];
+    }
+    return icempm::kNumTileCandidates;
+}
+
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen)
 {
     if (!p) return bad("null partition");

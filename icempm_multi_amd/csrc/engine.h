@@ -28,6 +28,10 @@ enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpd
 // Workgroups of the launch that handles the points a tile launch set aside.
 constexpr int kDeferredBlocks = 8;
 
+// Tile candidates of the autotuner: {tx, ty, lane spread, dynamic queues}.
+extern const int kTileCandidates[][4];
+extern const int kNumTileCandidates;
+
 // Grid buffers per scene.  A stepping call rotates through three: the tile
 // launch of substep k reads buffer k (momenta), accumulates into k+1 and
 // clears k+2 (mod 3).

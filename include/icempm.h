@@ -261,13 +261,18 @@ int icempm_partition_set_lane_map(icempm_partition *p, int lane_spread);
  * tile list, 1 = per-XCD atomic tile queues with stealing.  Set by the autotuner. */
 int icempm_partition_set_tile_schedule(icempm_partition *p, int dynamic);
 
-/* Project-written autotuner: times the fused G2P+P2G launch of every candidate
+/* The autotuner's candidate tile shapes (no reference counterpart): fills the
+ * first n of tx, ty, lane_spread (each may be null) and returns the number of
+ * candidates, K. */
+int icempm_tile_candidates(int *tx, int *ty, int *lane_spread, int n);
+
+/* Project-written autotuner: times the substep launch of every candidate
  * tile shape, each with a window drift margin of 0 and of 1, on the current
  * point set after half a frame of substeps (the state drifts from its sort as
  * it does mid-frame), keeps the fastest and restores the state.  Results (ms
- * per fused launch; -1 for a shape that does not fit) go to out_ms if
- * non-null: the candidates at margin 0, then at margin 1 (28 values).
- * *chosen = margin * 14 + candidate, the shape kept; the shape itself is read
+ * per substep launch; -1 for a shape that does not fit) go to out_ms if
+ * non-null: the K candidates at margin 0, then at margin 1 (2K values).
+ * *chosen = margin * K + candidate, the shape kept; the shape itself is read
  * with icempm_partition_get_tile_config. */
 int icempm_partition_autotune(icempm_partition *p, int n_trials, float *out_ms, int n_out, int *chosen);
 /* The tile shape, window drift margin and lane map in use (no reference counterpart). */

@@ -122,3 +122,15 @@ def test_no_device_calls_fail_cleanly():
         assert e.code < 0
     else:
         p.close()
+
+
+def test_tile_candidates():
+    """The autotuner's shapes: within the kernels' limits, the earlier powers of
+    two first (their results keep their indices), then the widths between
+    them that let a scene's tile count come close to whole rounds."""
+    c = capi.tile_candidates()
+    assert len(c) == capi.lib().icempm_tile_candidates(None, None, None, 0) >= 14
+    assert c[12] == (8, 64, 1) and c[11] == (16, 32, 1)
+    assert len(set(c)) == len(c)
+    assert all(1 <= tx <= 32 and ty in (8, 16, 32, 64) and ls in (0, 1, 2) for tx, ty, ls in c)
+    assert {(6, 64, 1), (7, 64, 1), (12, 32, 1)} <= set(c)
