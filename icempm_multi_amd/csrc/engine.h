@@ -22,6 +22,9 @@ namespace icempm {
 // The per-scene constants the kernels read, derived from the reference's
 // SimParams (parameters_sim.cpp:128-165) plus the invariants of the point loop.
 void scene_consts(const icempm_simparams &P, SceneConsts &c);
+// The scene's failure-surface table (SceneDev::fail_tab) from its constants,
+// with the reference's operations in its order (kernels.cuh:560-576, 598-619).
+void scene_fail_table(const SceneConsts &c, double *tab);
 
 enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kClsDeferred, kNumCls };
 

@@ -267,11 +267,32 @@ void scene_consts(const icempm_simparams &P, SceneConsts &c)
     }
 }
 
+void scene_fail_table(const SceneConsts &c, double *tab)
+{
+#pragma clang fp contract(off)
+    const double pmax = c.ics, pmin = -c.its;
+    for (int r = 0; r < kFailTab11; r++) {
+        // var3 = 1 + GrainVariability * 0.1 * (-10 + (grain + 4) % 11)
+        const double t = c.gv01 * (double)(-10 + (r - 10));
+        const double var3 = 1.0 + t;
+        const double qmax = c.iss * var3;
+        tab[2 * r] = (4 * qmax * qmax * c.b2) / ((pmax - pmin) * (pmax - pmin));   // mSq
+        tab[2 * r + 1] = c.q0a * qmax / (pmax - pmin);                               // q0
+    }
+    for (int s = 0; s < kFailTab30; s++) {
+        // var2 = 1 + GrainVariability * 0.033 * (-15 + (grain + 3) % 30)
+        const double t = c.gv033 * (double)(-15 + (s - 29));
+        const double var2 = 1.0 + t;
+        tab[2 * kFailTab11 + s] = -c.its2 * var2;                                     // pmin2
+    }
+}
+
 int Engine::build_scene_dev(int i, SceneDev &d) const
 {
     const SceneHost &s = scenes_[i];
     std::memset(&d, 0, sizeof d);
     scene_consts(s.prms, d.c);
+    scene_fail_table(d.c, d.fail_tab);
     const SceneConsts &c = d.c;
     d.pt_off = s.pt_off;
     d.n_pts = s.n_pts;

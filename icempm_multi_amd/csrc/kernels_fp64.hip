@@ -111,7 +111,8 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
     StressTerms terms;
     terms.valid = false;
     const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rb]);
-    if (!g2p_point(c, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, false, x, y, v, B, F, Jp, ud, pp, qq, e, terms)) {
+    if (!g2p_point(c, (const glb_d *)S.fail_tab, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, false, x, y, v, B, F, Jp,
+                   ud, pp, qq, e, terms)) {
         atomicMax(&a.err[sc], e);
         return;
     }
@@ -167,6 +168,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     extern __shared__ double lds_raw[];
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
+    __shared__ double ftab[kFailTab];   // failure-surface table of the tile's scene
 
     const int win_h = WHC > 0 ? WHC : a.win_h, win_w = a.win_w;
     const int WN = win_w * win_h;
@@ -204,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     const int q0 = nq == 8 ? (int)(blockIdx.x & 7) : 0;
     int qk = 0;                      // queues visited so far (dynamic)
     int w_static = xcd_block();
+    int tab_scene = -1;              // scene whose failure table ftab holds
     for (;;) {
         int w;
         if (dyn) {
@@ -240,6 +243,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         if (FROM_M) clk = next_clock(c, clk);
         bool record = false;
         if (DO_G2P) record = record_mode >= 0 ? record_mode != 0 : (clk.step % c.update_every) == 0;
+        // the failure-surface table of the tile's scene, kept while the scene
+        // stays the same (the barrier that ends a tile orders its reuse)
+        if (DO_G2P && ti.scene != tab_scene) {
+            for (int k = tid; k < kFailTab; k += kBlock) ftab[k] = S.fail_tab[k];
+            tab_scene = ti.scene;
+        }
 
         // point runs of the tile's columns
         if (tid < a.tx) {
@@ -411,8 +420,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                 // goes to the deferred list, which k_deferred handles after
                 // this launch: forming node velocities from global memory
                 // here would add to the point loop's registers.
-                if (!g2p_point(c, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, x, y, v, B, F, Jp, ud, pp,
-                               qq, e, terms)) {
+                if (!g2p_point(c, (const lds_d *)ftab, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, x, y, v, B,
+                               F, Jp, ud, pp, qq, e, terms)) {
                     if (e) {
                         err = max(err, e);
                     } else {

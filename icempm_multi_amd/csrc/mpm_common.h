@@ -43,6 +43,13 @@ struct ConstsF {
     float b2, bpmax;     // 1 + 2 beta, beta * pmax
 };
 
+// The grain-dependent terms of the failure surface (kernels.cuh:560-576) for
+// every remainder a 16-bit signed grain id leaves: entries 2r, 2r+1 hold mSq
+// and q0 for (grain + 4) % 11 = r - 10 (r in [0, 20]), entry 2*21 + s holds
+// pmin2 for (grain + 3) % 30 = s - 29 (s in [0, 58]).
+constexpr int kFailTab11 = 21, kFailTab30 = 59;
+constexpr int kFailTab = 2 * kFailTab11 + kFailTab30;
+
 // Everything a kernel needs to know about one scene's physics and geometry.
 struct SceneConsts {
     int gridX, gridY, halo, ncols;    // node column col = gx - xoff + halo, gx in [xoff-halo, xoff+gridX+halo)
@@ -83,6 +90,8 @@ struct SceneDev {
     int pad0;
     long long force_off;    // first of 2*n_ind_sub doubles in StepArgs::force_mem
     long long seg_off;      // first segment flag of the scene in each half of StepArgs::dirty
+    // grain-dependent terms of the FP64 failure surface (scene_fail_table)
+    double fail_tab[kFailTab];
 };
 
 // Device clock of a scene, advanced on the device once per substep so that a

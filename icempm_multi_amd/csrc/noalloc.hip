@@ -188,6 +188,7 @@ int icempm_substeps_noalloc(int device, const icempm_simparams *prms, size_t prm
     SceneDev d;
     std::memset(&d, 0, sizeof d);
     scene_consts(P, d.c);
+    scene_fail_table(d.c, d.fail_tab);
     d.pt_off = 0;
     d.n_pts = n_points;
     d.cap_pts = (int)pitch;

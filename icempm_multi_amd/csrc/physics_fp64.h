@@ -239,24 +239,24 @@ __device__ __forceinline__ void compute_pq(const SceneConsts &c, const double F[
     compute_pq(c, F, Je, p, q, st);
 }
 
-// CheckIfPointIsInsideFailureSurface + GetParametersForGrain (kernels.cuh:560-619)
-__device__ __forceinline__ bool failure(const SceneConsts &c, int grain, double p, double q)
+// CheckIfPointIsInsideFailureSurface + GetParametersForGrain (kernels.cuh:560-619).
+// The grain-dependent mSq, q0 and pmin2 come from the scene's table `tab`
+// (SceneDev::fail_tab, computed on the host with the same operations; an LDS
+// copy in the tile kernels), indexed by the two remainders of the grain id.
+template <class Tab>
+__device__ __forceinline__ bool failure(const SceneConsts &c, Tab tab, int grain, double p, double q)
 {
-    // the grain-independent terms come from the host (same operations)
-    const double var2 = 1.0 + c.gv033 * (-15 + (grain + 3) % 30);
-    const double var3 = 1.0 + c.gv01 * (-10 + (grain + 4) % 11);
-    const double pmax = c.ics;
-    const double qmax = c.iss * var3;
-    const double pmin2 = -c.its2 * var2;
-    const double mSq = (4 * qmax * qmax * c.b2) / c.dp2;
+    const int r11 = (grain + 4) % 11 + 10, r30 = (grain + 3) % 30 + 29;
+    const double pmin2 = tab[2 * kFailTab11 + r30];
     if (p < 0) {
         if (p < pmin2) return true;
-        const double q0 = c.q0a * qmax / c.dp;
+        const double q0 = tab[2 * r11 + 1];
         const double k = -q0 / pmin2;
         const double q_limit = k * (p - pmin2);
         return q > q_limit;
     }
-    const double y = c.b2 * q * q + mSq * (p + c.bpmax) * (p - pmax);
+    const double mSq = tab[2 * r11];
+    const double y = c.b2 * q * q + mSq * (p + c.bpmax) * (p - c.ics);
     return y > 0;
 }
 
@@ -304,8 +304,10 @@ __device__ __forceinline__ void drucker_prager(const SceneConsts &c, double F[4]
 // A = I + dt*C has been applied to F already.
 // The terms the stress shares are left in st (st.valid false when the
 // projection changed F or the point is liquid).
-__device__ __forceinline__ void constitutive(const SceneConsts &c, double F[4], double &Jp_inv, long long &ud,
-                                             const double A[4], double &p_out, double &q_out, StressTerms &st)
+template <class Tab>
+__device__ __forceinline__ void constitutive(const SceneConsts &c, Tab ftab, double F[4], double &Jp_inv,
+                                             long long &ud, const double A[4], double &p_out, double &q_out,
+                                             StressTerms &st)
 {
     st.valid = false;
     if (ud & kLiquid) {
@@ -317,7 +319,7 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, double F[4], 
     compute_pq(c, F, Je, p, q, st);
     p_out = p; q_out = q;
     if (!(ud & kCrushed))
-        if (failure(c, (short)ud, p, q)) ud |= kCrushed;
+        if (failure(c, ftab, (short)ud, p, q)) ud |= kCrushed;
     if (ud & kCrushed) {
         st.valid = false;
         drucker_prager(c, F, Jp_inv, Je, p, q);

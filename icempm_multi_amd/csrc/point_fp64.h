@@ -63,7 +63,9 @@ __device__ __forceinline__ int xcd_block()
 // here with that clock (node_velocity, no force).
 // defer_outside: if the stencil leaves the window, return false with e = 0
 // and change nothing (the caller handles the point later).
-__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, long long gpitch,
+// ftab: the scene's failure-surface table (SceneDev::fail_tab or a copy).
+template <class Tab>
+__device__ __forceinline__ bool g2p_point(const SceneConsts &c, Tab ftab, const glb_d *g, long long gpitch,
                                           const SceneClock *vclk, const lds_d *vw, int wc0, int wr0, int win_w,
                                           int win_h, bool defer_outside, double &x, double &y, double v[2],
                                           double B[4], double F[4], double &Jp, long long &ud, double &p_out,
@@ -173,7 +175,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const glb_d *g, 
     y += v[1] * c.dt;
     const double A[4] = {1.0 + c.dt * B[0], c.dt * B[1], c.dt * B[2], 1.0 + c.dt * B[3]};
     mul2(A, F, F);
-    constitutive(c, F, Jp, ud, A, p_out, q_out, st);
+    constitutive(c, ftab, F, Jp, ud, A, p_out, q_out, st);
     return true;
 }
 

@@ -170,6 +170,41 @@ def test_liquid_and_crushed_points_match_oracle(flags):
     assert np.abs(ref[3, :n][liq_now] - 1.0).max() > 1e-9      # Jp_inv of liquid points moved
 
 
+@pytest.mark.parametrize("flags", [0, im.STEP_DIRECT_P2G])
+def test_failure_surface_every_grain_remainder_matches_oracle(flags):
+    """The failure check (kernels.cuh:560-619) reads its grain terms from a
+    per-scene table indexed by (grain + 4) % 11 and (grain + 3) % 30 of the
+    grain id as a signed 16-bit value.  Points with grain ids over the whole
+    16-bit range (negative as shorts above 32767, so negative remainders) and
+    strains log-uniform from 3e-7 to 1e-3, which put them on both sides of the
+    surface (30% crush in the oracle, of both signs of grain), crush exactly
+    where the oracle's do."""
+    P, h, n = small_block(seed=0x1CE0203)
+    with_motion(P, h, n, seed=10)
+    r = np.random.default_rng(12)
+    ud = h[0, :n].view(np.int64)
+    ud[:] = (ud & ~np.int64(0xffff)) | r.integers(0, 65536, n)
+    s = 10.0 ** r.uniform(-6.5, -3.0, n)
+    h[8:12, :n] = np.array([1.0, 0, 0, 1.0])[:, None] + s * r.normal(size=(4, n))
+    ref = h.copy()
+    part = make_partition(P, h, n)
+    if not flags & im.STEP_DIRECT_P2G:
+        part.sort_points()
+        O.sort(P, ref, n)
+    part.step(2, flags)
+    t, step, _, err = O.substeps(P, ref, n, 2, 0.0, 0)
+    assert err == 0 and part.error_code == 0
+    out = download(part, h)
+    u_ref = ref[0, :n].view(np.int64)
+    assert np.array_equal(out[0, :n].view(np.int64), u_ref)
+    crushed = (u_ref & STATUS_CRUSHED) != 0
+    grain = (u_ref & 0xffff).astype(np.int16)
+    assert 0.1 * n < crushed.sum() < 0.9 * n, crushed.sum()
+    for sel in (grain < 0, grain >= 0):
+        assert crushed[sel].any() and (~crushed[sel]).any()
+    assert_close(out, ref, n, rtol=1e-9, what=f"failure surface flags={flags}")
+
+
 @pytest.mark.parametrize("side", ["left", "right"])
 def test_walls_and_vmax_clamp_match_oracle(side):
     """The grid update's side-wall rules (kernels.cuh:290-294) with the block
