@@ -103,7 +103,7 @@ _SIGS = {
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
 
-KERNEL_CLASSES = ("clock", "zero", "p2g", "fused_g2p_p2g", "g2p", "update", "sort", "deferred")
+KERNEL_CLASSES = ("clock", "zero", "p2g", "fused_g2p_p2g", "g2p", "update", "sort")
 
 
 def lib():

@@ -273,7 +273,7 @@ int icempm_partition_get_timings(icempm_partition *p, float *out, int n)
         t[0] = (float)ms[icempm::kClsP2G];
         t[2] = (float)ms[icempm::kClsUpdate];
         // the binned stepping folds the grid update into its G2P launches
-        t[3] = (float)(ms[icempm::kClsG2P] + ms[icempm::kClsFused] + ms[icempm::kClsDeferred]);
+        t[3] = (float)(ms[icempm::kClsG2P] + ms[icempm::kClsFused]);
         for (int k = 0; k < icempm::kNumCls; k++)
             if (k != icempm::kClsSort) t[6] += (float)ms[k];
         t[7] = (float)ms[icempm::kClsSort];

@@ -26,10 +26,7 @@ void scene_consts(const icempm_simparams &P, SceneConsts &c);
 // with the reference's operations in its order (kernels.cuh:560-576, 598-619).
 void scene_fail_table(const SceneConsts &c, double *tab);
 
-enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kClsDeferred, kNumCls };
-
-// Workgroups of the launch that handles the points a tile launch set aside.
-constexpr int kDeferredBlocks = 8;
+enum KernelClass { kClsClock = 0, kClsZero, kClsP2G, kClsFused, kClsG2P, kClsUpdate, kClsSort, kNumCls };
 
 // Tile candidates of the autotuner: {tx, ty, lane spread, dynamic queues}.
 extern const int kTileCandidates[][4];

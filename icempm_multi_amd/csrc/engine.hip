@@ -5,8 +5,8 @@
 // Binned pipeline, three accumulation grids per scene (indices mod 3):
 //   [P2G -> 0]; [update 0 + G2P from 0 + P2G -> 1, clear 2]; [update 1 + G2P
 //   from 1 + P2G -> 2, clear 0]; ...; [update + G2P of the last substep]
-// one tile launch per substep (each followed by the deferred-point launch),
-// so each substep after the first reads the points once and writes them once.
+// one tile launch per substep, so each substep after the first reads the
+// points once and writes them once.
 // Unbinned fallback (before the first sort): zero A; P2G direct; update(A); G2P direct.
 #include <hip/hip_runtime.h>
 
@@ -1002,7 +1002,7 @@ int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *ce
 // A call of nsub substeps is a sequence of segments of one kind of unit:
 //   binned, fused:   Head [P2G(b)], nsub-1 x Chain, Tail
 //     Chain [grid update + G2P from grid b, P2G into b+1, clear b+2; the
-//            points set aside], then b = b+1 (mod 3)
+//            points set aside, by its last workgroup], then b = b+1 (mod 3)
 //     Tail  [grid update + G2P from grid b, clear b+2; the points set aside]
 //   binned, unfused: nsub x Unfused [P2G(b), update(b), G2P(b)], b = b^1 (0, 1)
 //   unbinned:        nsub x Direct [zero, P2G direct, update, G2P direct]
@@ -1036,9 +1036,6 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
             timed_begin(timed);
             HIPCHK(LAUNCH_PREC(launch_tiles, au, mode, b, w, z, -1, tile_blocks(mode), stream_));
             timed_end(timed, kind == kUnitChain ? kClsFused : kClsG2P);
-            timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_deferred, au, mode, b, w, kDeferredBlocks, stream_));
-            timed_end(timed, kClsDeferred);
             b = w;
             cs ^= 1;
             continue;

@@ -28,9 +28,6 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbu
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
-// The points a kTileFromMomenta launch of `mode` set aside (their stencil left
-// the LDS window), from grid[rbuf] (momenta) into grid[wbuf].
-hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 // Workgroups of the tile kernel for this mode and window that one CU holds at
 // once (registers and LDS); 0 if the runtime cannot tell.
@@ -44,9 +41,6 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbu
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
-// The points a kTileFromMomenta launch of `mode` set aside (their stencil left
-// the LDS window), from grid[rbuf] (momenta) into grid[wbuf].
-hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 // Workgroups of the tile kernel for this mode and window that one CU holds at
 // once (registers and LDS); 0 if the runtime cannot tell.

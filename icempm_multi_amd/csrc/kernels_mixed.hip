@@ -170,6 +170,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
     const int q0 = nq == 8 ? (int)(blockIdx.x & 7) : 0;
     int qk = 0;                      // queues visited so far (dynamic)
     int w_static = xcd_block();
+    __shared__ int s_defer;          // this workgroup set points aside
+    if (tid == 0) s_defer = 0;
     for (;;) {
         int w;
         if (dyn) {
@@ -341,13 +343,14 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                 float pp, qq;
                 unsigned e;
                 // a point whose G2P stencil left the window goes to the
-                // deferred list, which k_deferred handles after this launch
+                // deferred list, handled by the launch's last workgroup
                 if (!g2p_point(c, cf, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, xq, yq, v, C, D, Jm1, ud,
                                pp, qq, e, (unsigned)pt, (unsigned)step, terms)) {
                     if (e) {
                         err = max(err, e);
                     } else {
                         const int slot = atomicAdd(a.defer_count, 1);
+                        s_defer = 1;
                         a.defer_pts[slot] = (int)pt;
                         a.defer_scene[slot] = ti.scene;
                     }
@@ -417,35 +420,33 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
         if (atomicAdd(a.tile_queue + 8, 1) == (int)gridDim.x - 1)
             for (int k = 0; k < 9; k++) a.tile_queue[k] = 0;
     }
-}
-
-// ---------------------------------------------------------------------------
-// The points a kTileFromMomenta launch set aside (their G2P stencil left the
-// tile's LDS window), handled by the launch that follows it: the point list
-// needs every workgroup of that launch to have finished.  A few workgroups
-// suffice (points drift out of their window rarely); the last to finish
-// empties the list.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_deferred(StepArgs a, int p2g, int rbuf, int wbuf)
-{
-    const int n = *a.defer_count;
-    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
-        deferred_point(a, p2g != 0, rbuf, wbuf, k);
-    __shared__ bool last;
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(a.defer_count + 1, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        a.defer_count[0] = 0;
-        a.defer_count[1] = 0;
+    if (FROM_M) {
+        // The points this launch set aside are handled by the last workgroup
+        // to finish (their G2P reads grid[rbuf], complete before the launch;
+        // their P2G adds into grid[wbuf] with atomics).  A workgroup that set
+        // some aside publishes the list entries with a device-scope fence
+        // before it counts itself done; the others (all of them in a step
+        // where no point left its window) need none.
+        __shared__ int s_last;
+        __syncthreads();
+        if (tid == 0) {
+            if (s_defer) __threadfence();
+            s_last = atomicAdd(a.defer_count + 1, 1) == (int)gridDim.x - 1;
+        }
+        __syncthreads();
+        if (s_last) {
+            const int n = __hip_atomic_load(a.defer_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (n > 0) {
+                __threadfence();   // the entries the other workgroups published
+                for (int k = tid; k < n; k += kBlock) deferred_point(a, DO_P2G, rbuf, wbuf, k);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                a.defer_count[0] = 0;
+                a.defer_count[1] = 0;
+            }
+        }
     }
-}
-
-hipError_t launch_deferred(const StepArgs &a, int mode, int rbuf, int wbuf, int n_blocks, hipStream_t s)
-{
-    if (n_blocks <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_deferred, dim3(n_blocks), dim3(kBlock), 0, s, a, (mode & kTileP2G) ? 1 : 0, rbuf, wbuf);
-    return hipGetLastError();
 }
 
 size_t tile_lds_bytes(const StepArgs &a, int mode)

@@ -143,9 +143,9 @@ struct StepArgs {
     int *tile_queue;          // 8 queue counters + 1 finished-block counter, zero between launches
     int *dirty;               // segment flags: [buffer 0 of all scenes][buffer 1 ...][buffer 2 ...]
     long long dirty_stride;   // segments per buffer (all scenes)
-    // Points a kTileFromMomenta launch sets aside for k_deferred (one slot per
-    // point of the point space; defer_count[0] = entries, [1] = k_deferred's
-    // finished workgroups; both zero between launches).
+    // Points a kTileFromMomenta launch sets aside for its last workgroup (one
+    // slot per point of the point space; defer_count[0] = entries, [1] = the
+    // launch's finished workgroups; both zero between launches).
     int *defer_pts;
     int *defer_scene;
     int *defer_count;

@@ -120,9 +120,9 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, Tab ftab, const 
             column(i, ax, ay, bx, by_);
         }
     } else if (vclk) {
-        // momenta in global memory (the deferred-point kernel, which has
-        // registers to spare): the 27 loads of the 9 nodes issued together,
-        // then each node's velocity formed here
+        // momenta in global memory (the points a launch set aside, handled
+        // at its end): the 27 loads of the 9 nodes issued together, then
+        // each node's velocity formed here
         double gm[9], gpx[9], gpy[9];
 #pragma unroll
         for (int n = 0; n < 9; n++) {

@@ -186,8 +186,8 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
             column(i, ax, ay, bx_, by_);
         }
     } else if (vclk) {
-        // compensated momenta in global memory (the deferred-point kernel,
-        // which has registers to spare): the loads of the 9 nodes issued
+        // compensated momenta in global memory (the points a launch set
+        // aside, handled at its end): the loads of the 9 nodes issued
         // together, then each node's velocity formed here
         float gl[9][6];
 #pragma unroll

@@ -210,9 +210,8 @@ int icempm_partition_get_timings(icempm_partition *p, float *out, int n);
  * recorded with hipEvents around each launch when ICEMPM_STEP_TIMINGS is set:
  * class 0 clock, 1 zero-fill, 2 P2G, 3 fused G2P+P2G (the grid update folded
  * in), 4 G2P (the last of a binned call also folds the grid update in), 5 grid
- * update (unbinned and unfused stepping, and the phase calls), 6 sort, 7 the
- * points a fused or last G2P launch set aside (their stencil left the tile's
- * window).  ms[k] = summed milliseconds, count[k] = launches. */
+ * update (unbinned and unfused stepping, and the phase calls), 6 sort.
+ * ms[k] = summed milliseconds, count[k] = launches. */
 int icempm_partition_get_kernel_stats(icempm_partition *p, double *ms, long long *count, int n);
 
 /* Frame-level substep loop of icy::Model::Step (model.cpp:38-54) run on the

@@ -239,9 +239,8 @@ def test_walls_and_vmax_clamp_match_oracle(side):
 @pytest.mark.parametrize("graph", [0, im.STEP_GRAPH])
 def test_sampled_timings_count_launches(graph):
     """Events around the launches of substeps 0, 32, 64 of a 70-substep call:
-    the P2G of substep 0, three fused launches (the grid update folded in) and
-    the deferred-point launch after each; a replay of the timed graph adds as
-    many again."""
+    the P2G of substep 0 and three fused launches (the grid update folded in,
+    one launch per substep); a replay of the timed graph adds as many again."""
     P, h, n = small_block(seed=0x1CE0201)
     part = make_partition(P, h, n)
     part.sort_points()
@@ -249,12 +248,12 @@ def test_sampled_timings_count_launches(graph):
     flags = graph | im.STEP_TIMINGS | im.STEP_SAMPLED
     part.step(70, flags)
     st = part.kernel_stats()
-    assert st["p2g"][1] == 1 and st["fused_g2p_p2g"][1] == 3 and st["deferred"][1] == 3
+    assert st["p2g"][1] == 1 and st["fused_g2p_p2g"][1] == 3
     assert st["update"][1] == 0 and st["g2p"][1] == 0
-    assert all(st[k][0] > 0 for k in ("p2g", "fused_g2p_p2g", "deferred"))
+    assert all(st[k][0] > 0 for k in ("p2g", "fused_g2p_p2g"))
     part.step(70, flags)
     st = part.kernel_stats()
-    assert st["p2g"][1] == 2 and st["fused_g2p_p2g"][1] == 6 and st["deferred"][1] == 6
+    assert st["p2g"][1] == 2 and st["fused_g2p_p2g"][1] == 6
     part.step(70, im.STEP_GRAPH)   # untimed: no more events
     assert part.kernel_stats()["fused_g2p_p2g"][1] == 6
     assert part.error_code == 0
