@@ -98,6 +98,7 @@ _SIGS = {
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "icempm_batch_get_tile_config": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                                C.POINTER(C.c_int)]),
+    "icempm_batch_set_tile_shape": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     "icempm_batch_get_graph_stats": (C.c_int, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
 }
 
@@ -443,6 +444,9 @@ class Batch:
         chosen = C.c_int(-1)
         _check(lib().icempm_batch_autotune(self._h, n_trials, out, k, C.byref(chosen)))
         return chosen.value, list(out)
+
+    def set_tile_shape(self, tx, ty, margin=1):
+        _check(lib().icempm_batch_set_tile_shape(self._h, tx, ty, margin))
 
     def tile_config(self):
         v = [C.c_int() for _ in range(4)]

@@ -542,6 +542,12 @@ int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_ou
     return guarded([&] { return b->eng->autotune(n_trials, out_ms, n_out, chosen); });
 }
 
+int icempm_batch_set_tile_shape(icempm_batch *b, int tx, int ty, int margin)
+{
+    if (!b) return bad("null batch");
+    return guarded([&] { return b->eng->set_tile_shape(tx, ty, margin); });
+}
+
 int icempm_batch_get_tile_config(icempm_batch *b, int *tx, int *ty, int *margin, int *lane_spread)
 {
     if (!b) return bad("null batch");

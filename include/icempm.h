@@ -314,6 +314,9 @@ int icempm_batch_get_scene_count(icempm_batch *b, int *n_scenes, long long *n_po
 int icempm_batch_autotune(icempm_batch *b, int n_trials, float *out_ms, int n_out, int *chosen);
 /* As the partition functions of the same names. */
 int icempm_batch_get_tile_config(icempm_batch *b, int *tx, int *ty, int *margin, int *lane_spread);
+/* icempm_partition_set_tile_shape for every scene of the batch (no reference
+ * counterpart): pins the shape the autotuner would choose. */
+int icempm_batch_set_tile_shape(icempm_batch *b, int tx, int ty, int margin);
 int icempm_batch_get_graph_stats(icempm_batch *b, long long *captures, int *cached);
 /* Average device time per substep (ms) of the last icempm_batch_step call. */
 int icempm_batch_last_step_ms(icempm_batch *b, float *ms_per_substep);

@@ -53,3 +53,37 @@ def test_window_margin_matches_oracle(precision, shape, margin):
     else:
         assert rel(out[4:6, :n], ref[4:6, :n]) <= 1e-6
         assert rel(out[6:8, :n], ref[6:8, :n]) <= 1e-4
+
+
+def test_batch_points_set_aside_match_oracle():
+    """Three translating blocks of different speeds in one batch, margin-0
+    windows: in every substep many points of every scene leave their window
+    and go to the launch's list of points set aside, which its last workgroup
+    handles (each entry carries its scene).  Every scene matches its own FP64
+    oracle run."""
+    bag = []
+    for k, (vx, vy) in enumerate(((50.0, 20.0), (-40.0, 10.0), (30.0, -25.0))):
+        P, h, n = scenes.block_scene(nx=40 + 8 * k, ny=10 + 2 * k, grid=(96, 48), n_grains=30, seed=0x1CE0C10 + k)
+        h[6, :n] = vx + 0.5 * np.sin(h[4, :n])
+        h[7, :n] = vy + 0.3 * np.cos(h[5, :n])
+        bag.append((P, h, n))
+    b = im.Batch(0, im.PRECISION_FP64)
+    for P, h, n in bag:
+        b.add_scene(P, h, n)
+    b.commit()
+    b.set_tile_shape(8, 32, 0)
+    b.step(NSUB // 2, im.STEP_RESORT)
+    b.step(NSUB - NSUB // 2, im.STEP_GRAPH)
+    assert b.error_code == 0
+    for i, (P, h, n) in enumerate(bag):
+        ref = h.copy()
+        O.sort(P, ref, n)
+        _, _, _, err = O.substeps(P, ref, n, NSUB, 0.0, 0)
+        assert err == 0
+        out = np.zeros_like(h)
+        b.transfer_from_device(i, out)
+        for lines in ((4, 6), (6, 8), (8, 12), (12, 16)):
+            sc = np.abs(ref[lines[0]:lines[1], :n]).max()
+            d = np.abs(out[lines[0]:lines[1], :n] - ref[lines[0]:lines[1], :n]).max()
+            assert d <= 1e-9 * sc, (i, lines, d, sc)
+    b.close()
