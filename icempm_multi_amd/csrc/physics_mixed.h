@@ -67,7 +67,8 @@ __device__ __forceinline__ void stress_terms(const ConstsF &c, const float D[4],
     float Jm1;
     strain_measures(D, Jm1, st.d0, st.d1);
     const float J = 1.f + Jm1;
-    st.a = c.mu / J;
+    // v_rcp_f32 (1 ulp) in place of the IEEE divide's ten-instruction sequence
+    st.a = c.mu * __builtin_amdgcn_rcpf(J);
     st.k = 0.5f * c.kappa * Jm1 * (J + 1.f);
     st.valid = true;
 }
@@ -97,11 +98,12 @@ __device__ __forceinline__ void water(const ConstsF &c, float q, float tau[4])
 }
 
 // ComputePQ (kernels.cuh:547-557) from the stress terms: p = -k,
-// q = sqrt(2) (mu/J) |dev(F F^T)|.
+// q = sqrt(2) (mu/J) |dev(F F^T)|, with the 1-ulp v_sqrt_f32 (q only meets
+// the failure test, which FP32 rounding already moves by more than an ulp).
 __device__ __forceinline__ void compute_pq(const StressTerms &st, float &p, float &q)
 {
     p = -st.k;
-    q = kCoeff1f * st.a * sqrtf(2.f * (st.d0 * st.d0 + st.d1 * st.d1));
+    q = kCoeff1f * st.a * __builtin_amdgcn_sqrtf(2.f * (st.d0 * st.d0 + st.d1 * st.d1));
 }
 
 // CheckIfPointIsInsideFailureSurface (kernels.cuh:560-619) in FP32
@@ -114,8 +116,8 @@ __device__ __forceinline__ bool failure(const ConstsF &c, int grain, float p, fl
     if (p < 0) {
         if (p < pmin2) return true;
         const float q0 = c.q0c * qmax;
-        const float q_limit = -q0 / pmin2 * (p - pmin2);
-        return q > q_limit;
+        // q > -q0/pmin2 (p - pmin2), multiplied through by -pmin2 > 0
+        return q * -pmin2 > q0 * (p - pmin2);
     }
     const float mSq = c.msq_c * (qmax * qmax);
     const float y = c.b2 * q * q + mSq * (p + c.bpmax) * (p - c.ics);
