@@ -207,11 +207,40 @@ __global__ void __launch_bounds__(kBlock) k_tile_emit(StepArgs a, const int *til
     }
     const SceneDev &S = a.scenes[lo];
     const int local = t - tile_off[lo];
+    const int col0 = (local / S.n_tiles_y) * a.tx, row0 = (local % S.n_tiles_y) * a.ty;
+    const int rend = min(row0 + a.ty, S.c.gridY);
+    // bounding box of the cells that hold points; cell_start is nondecreasing,
+    // so the first and last occupied rows of a column are found by bisection
+    int c_lo = a.tx, c_hi = -1, r_lo = a.ty, r_hi = -1;
+    for (int k = 0; k < a.tx; k++) {
+        const int col = col0 + k;
+        if (col >= S.c.ncols) break;
+        const int *cs = a.cell_start + S.cell_off + (long long)col * S.c.gridY;
+        const int s0 = cs[row0], s1 = cs[rend];
+        if (s1 == s0) continue;
+        c_lo = min(c_lo, k);
+        c_hi = k;
+        int l = row0, h = rend - 1;           // first row r with cs[r + 1] > s0
+        while (l < h) {
+            const int m = (l + h) >> 1;
+            if (cs[m + 1] > s0) h = m;
+            else l = m + 1;
+        }
+        r_lo = min(r_lo, l - row0);
+        l = row0;
+        h = rend - 1;                         // last row r with cs[r] < s1
+        while (l < h) {
+            const int m = (l + h + 1) >> 1;
+            if (cs[m] < s1) l = m;
+            else h = m - 1;
+        }
+        r_hi = max(r_hi, l - row0);
+    }
     TileItem it;
     it.scene = lo;
-    it.col0 = (local / S.n_tiles_y) * a.tx;
-    it.row0 = (local % S.n_tiles_y) * a.ty;
-    it.npts = tile_n[t];
+    it.col0 = col0 + c_lo;
+    it.row0 = row0 + r_lo;
+    it.dims = (c_hi - c_lo + 1) | ((r_hi - r_lo + 1) << 16);
     tiles[tile_pos[t]] = it;
 }
 

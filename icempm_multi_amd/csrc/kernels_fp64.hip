@@ -111,7 +111,7 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
     StressTerms terms;
     terms.valid = false;
     const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rb]);
-    if (!g2p_point(c, (const glb_d *)S.fail_tab, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, false, x, y, v, B, F, Jp,
+    if (!g2p_point(c, (const glb_d *)S.fail_tab, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, 0, false, x, y, v, B, F, Jp,
                    ud, pp, qq, e, terms)) {
         atomicMax(&a.err[sc], e);
         return;
@@ -149,9 +149,11 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
 // vector registers across the loop (242 VGPRs, 2 waves per SIMD, before).
 // ---------------------------------------------------------------------------
 //
-// WHC is the window height (ty + 2 + 2*margin) when fixed at compile time for
-// the common tile shapes (0: read from the arguments).  A compile-time height
-// turns the stencil's LDS offsets into instruction immediates.
+// WHC is the LDS row stride, the height of the largest window (ty + 2 +
+// 2*margin), when fixed at compile time for the common tile shapes (0: read
+// from the arguments).  A compile-time stride turns the stencil's LDS offsets
+// into instruction immediates.  A tile's window covers the box of its cells
+// that held points at the last sort (TileItem), so it may be smaller.
 //
 // With kTileFromMomenta (the chain of a stepping call) grid[rbuf] holds the
 // momenta of the last P2G: the staging forms each window node's velocity
@@ -170,17 +172,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     __shared__ int run_pre[kMaxTX + 1];
     __shared__ double ftab[kFailTab];   // failure-surface table of the tile's scene
 
-    const int win_h = WHC > 0 ? WHC : a.win_h, win_w = a.win_w;
-    const int WN = win_w * win_h;
+    // The LDS regions hold the largest window, a.win_w columns of WS nodes; a
+    // tile's own window (TileWindow) takes its first columns and rows.
+    const int WS = WHC > 0 ? WHC : a.win_h;
+    const int WNmax = a.win_w * WS;
     lds_d *lds = (lds_d *)lds_raw;
-    lds_d *vw = lds;                            // 2*WN: (vx, vy) per window node
-    // Two accumulation windows of 3*WN, (m, px, py) per node: even lanes add
+    lds_d *vw = lds;                            // 2*WNmax: (vx, vy) per window node
+    // Two accumulation windows of 3*WNmax, (m, px, py) per node: even lanes add
     // into the first, odd lanes into the second.  The adjacent lanes of a
     // 16-lane row hold points 4 apart in the sorted order, which share a cell
     // (and so every LDS address) whenever the cell holds more than 4 points;
     // with a window each they never collide.  The flush adds the two.
-    lds_d *acc = lds + (DO_G2P ? 2 * WN : 0);
-    lds_d *const acc_lane = acc + ((threadIdx.x & 1) ? 3 * WN : 0);
+    lds_d *acc = lds + (DO_G2P ? 2 * WNmax : 0);
+    lds_d *const acc_lane = acc + ((threadIdx.x & 1) ? 3 * WNmax : 0);
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(double *)a.pts, a.pitch};
@@ -234,7 +238,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         const SceneDev &S = a.scenes[ti.scene];
         const SceneConsts c = uniform_copy(S.c);
         const int gridY = c.gridY;
-        const int wc0 = ti.col0 - 1 - a.margin, wr0 = ti.row0 - 1 - a.margin;
+        const TileWindow W(ti, a.margin);
+        const int wc0 = W.c0, wr0 = W.r0, win_w = W.w, win_h = W.h, WN = W.nodes();
+        const int ncol = ti.dims & 0xffff, nrow = ti.dims >> 16;
         const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rbuf]);
         glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
@@ -252,14 +258,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             tab_scene = ti.scene;
         }
 
-        // point runs of the tile's columns
-        if (tid < a.tx) {
+        // point runs of the columns of the tile's occupied box
+        if (tid < ncol) {
             const int col = ti.col0 + tid;
             long long s0 = 0, s1 = 0;
             if (col < c.ncols) {
                 const long long base = S.cell_off + (long long)col * gridY;
                 s0 = a.cell_start[base + ti.row0];
-                s1 = a.cell_start[base + min(ti.row0 + a.ty, gridY)];
+                s1 = a.cell_start[base + min(ti.row0 + nrow, gridY)];
             }
             run_start[tid] = s0;
             run_pre[tid + 1] = (int)(s1 - s0);
@@ -271,10 +277,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             constexpr int kU = 4;
             for (int i0 = tid; i0 < WN; i0 += kU * kBlock) {
                 double m[kU], px[kU], py[kU];
+                int qc[kU], qr[kU];
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
                     const int i = i0 + u * kBlock;
-                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    qc[u] = W.col_of(i);
+                    qr[u] = i - qc[u] * win_h;
+                    const int col = wc0 + qc[u], row = wr0 + qr[u];
                     const bool in = i < WN && col >= 0 && col < c.ncols && row >= 0 && row < gridY;
                     const long long gi = (long long)col * gridY + row;
                     m[u] = in ? (double)gr[gi] : 0.0;
@@ -285,38 +294,41 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                 for (int u = 0; u < kU; u++) {
                     const int i = i0 + u * kBlock;
                     if (i >= WN) break;
-                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
                     double vx = 0, vy = 0;
-                    if (m[u] != 0) node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, m[u], px[u], py[u], vx, vy);
-                    vw[2 * i] = vx;
-                    vw[2 * i + 1] = vy;
+                    if (m[u] != 0)
+                        node_velocity(c, clk, nullptr, wc0 + qc[u] - c.halo + c.xoff, wr0 + qr[u], m[u], px[u], py[u],
+                                      vx, vy);
+                    const int li = qc[u] * WS + qr[u];
+                    vw[2 * li] = vx;
+                    vw[2 * li + 1] = vy;
                 }
             }
         }
         for (int i = tid; i < WN; i += kBlock) {
+            const int qc = W.col_of(i), qr = i - qc * win_h, li = qc * WS + qr;
             if (DO_G2P && !FROM_M) {
-                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                const int col = wc0 + qc, row = wr0 + qr;
                 double vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
                     const long long gi = (long long)col * gridY + row;
                     vx = gr[gp + gi];
                     vy = gr[2 * gp + gi];
                 }
-                vw[2 * i] = vx;
-                vw[2 * i + 1] = vy;
+                vw[2 * li] = vx;
+                vw[2 * li + 1] = vy;
             }
             if (DO_P2G) {
 #pragma unroll
-                for (int q = 0; q < 3; q++) acc[3 * i + q] = acc[3 * WN + 3 * i + q] = 0;
+                for (int q = 0; q < 3; q++) acc[3 * li + q] = acc[3 * WNmax + 3 * li + q] = 0;
             }
         }
         __syncthreads();
         if (tid == 0) {
             run_pre[0] = 0;
-            for (int k = 0; k < a.tx; k++) run_pre[k + 1] += run_pre[k];
+            for (int k = 0; k < ncol; k++) run_pre[k + 1] += run_pre[k];
         }
         __syncthreads();
-        const int total = run_pre[a.tx];
+        const int total = run_pre[ncol];
 
         // points of the tile, strided over the workgroup: consecutive lanes take
         // consecutive points of a run, so every SoA access is coalesced
@@ -360,9 +372,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             }
             const int lx = st.bxc - wc0, ly = st.by - wr0;
             if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
-                lds_d *const p0 = acc_lane + 3 * (lx * win_h + ly);
+                lds_d *const p0 = acc_lane + 3 * (lx * WS + ly);
                 st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
-                    lds_d *const p = p0 + 3 * (i * win_h + j);
+                    lds_d *const p = p0 + 3 * (i * WS + j);
                     lds_add(p, dm);
                     lds_add(p + 1, dpx);
                     lds_add(p + 2, dpy);
@@ -370,7 +382,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
             } else {
                 st.nodes(c, [&](int i, int j, double dm, double dpx, double dpy) {
                     if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
-                        lds_d *const p = acc_lane + 3 * ((lx + i) * win_h + ly + j);
+                        lds_d *const p = acc_lane + 3 * ((lx + i) * WS + ly + j);
                         lds_add(p, dm);
                         lds_add(p + 1, dpx);
                         lds_add(p + 2, dpy);
@@ -422,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
                 // goes to the deferred list, which the launch's last
                 // workgroup handles at its end: forming node velocities from
                 // global memory here would add to the point loop's registers.
-                if (!g2p_point(c, (const lds_d *)ftab, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, x, y, v, B,
+                if (!g2p_point(c, (const lds_d *)ftab, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, WS, FROM_M, x, y, v, B,
                                F, Jp, ud, pp, qq, e, terms)) {
                     if (e) {
                         err = max(err, e);
@@ -448,14 +460,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
         if (DO_P2G) {
             __syncthreads();
             for (int i = tid; i < WN; i += kBlock) {
-                const double m = acc[3 * i] + acc[3 * WN + 3 * i];
+                const int qc = W.col_of(i), qr = i - qc * win_h, li = qc * WS + qr;
+                const double m = acc[3 * li] + acc[3 * WNmax + 3 * li];
                 if (m == 0) continue;
-                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                const int col = wc0 + qc, row = wr0 + qr;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
                 glb_add(gw + gi, m);
-                glb_add(gw + gp + gi, acc[3 * i + 1] + acc[3 * WN + 3 * i + 1]);
-                glb_add(gw + 2 * gp + gi, acc[3 * i + 2] + acc[3 * WN + 3 * i + 2]);
+                glb_add(gw + gp + gi, acc[3 * li + 1] + acc[3 * WNmax + 3 * li + 1]);
+                glb_add(gw + 2 * gp + gi, acc[3 * li + 2] + acc[3 * WNmax + 3 * li + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }

@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     unsigned e;
     StressTerms terms;
     if (!g2p_point(c, (const glb_d *)S.fail_tab, (const glb_d *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr,
-                   nullptr, 0, 0, 0, 0, false, x, y, v, B, F, Jp, ud, pp, qq, e, terms)) {
+                   nullptr, 0, 0, 0, 0, 0, false, x, y, v, B, F, Jp, ud, pp, qq, e, terms)) {
         atomicMax(&a.err[scene], e);
         return;
     }

@@ -95,7 +95,7 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
     StressTerms terms;
     terms.valid = false;
     const glb_f *gr = (const glb_f *)(a.grid_mem + S.grid_off[rb]);
-    if (!g2p_point(c, cf, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, false, xq, yq, v, C, D, Jm1, ud, pp, qq, e, pt,
+    if (!g2p_point(c, cf, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, 0, false, xq, yq, v, C, D, Jm1, ud, pp, qq, e, pt,
                    (unsigned)clk.step, terms)) {
         atomicMax(&a.err[sc], e);
         return;
@@ -123,9 +123,10 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
     });
 }
 
-// 4 waves per SIMD (<= 128 VGPRs).  WHC: the window height when fixed at
-// compile time (0: read from the arguments), as in the FP64 form; so is the
-// folded grid update of kTileFromMomenta.
+// 4 waves per SIMD (<= 128 VGPRs).  WHC: the LDS row stride (the largest
+// window's height) when fixed at compile time (0: read from the arguments),
+// as in the FP64 form; so are the tile windows and the folded grid update of
+// kTileFromMomenta.
 template <int MODE, int WHC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
@@ -137,14 +138,16 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
     __shared__ long long run_start[kMaxTX];
     __shared__ int run_pre[kMaxTX + 1];
 
-    const int win_h = WHC > 0 ? WHC : a.win_h, win_w = a.win_w;
-    const int WN = win_w * win_h;
-    // Two FP64 accumulation windows of 3*WN, (m, px, py) per node, one for
+    // The LDS regions hold the largest window, a.win_w columns of WS nodes; a
+    // tile's own window (TileWindow) takes its first columns and rows.
+    const int WS = WHC > 0 ? WHC : a.win_h;
+    const int WNmax = a.win_w * WS;
+    // Two FP64 accumulation windows of 3*WNmax, (m, px, py) per node, one for
     // even and one for odd lanes as in the FP64 form, then the FP32 velocity
     // window.
     lds_d *acc = (lds_d *)lds_raw;
-    lds_d *const acc_lane = acc + ((threadIdx.x & 1) ? 3 * WN : 0);
-    lds_f *vw = (lds_f *)(acc + (DO_P2G ? 6 * WN : 0));   // 2*WN: (vx, vy) per window node
+    lds_d *const acc_lane = acc + ((threadIdx.x & 1) ? 3 * WNmax : 0);
+    lds_f *vw = (lds_f *)(acc + (DO_P2G ? 6 * WNmax : 0));   // 2*WNmax: (vx, vy) per window node
     const int tid = threadIdx.x;
     const int ntiles = *a.n_tiles;
     const PtRef R{(unsigned *)a.pts, a.pitch};
@@ -198,7 +201,9 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
         const SceneConsts c = uniform_copy(S.c);
         const ConstsF cf = to_float(c);
         const int gridY = c.gridY;
-        const int wc0 = ti.col0 - 1 - a.margin, wr0 = ti.row0 - 1 - a.margin;
+        const TileWindow W(ti, a.margin);
+        const int wc0 = W.c0, wr0 = W.r0, win_w = W.w, win_h = W.h, WN = W.nodes();
+        const int ncol = ti.dims & 0xffff, nrow = ti.dims >> 16;
         const glb_f *gr = (const glb_f *)(a.grid_mem + S.grid_off[rbuf]);
         glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
@@ -211,13 +216,14 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
         const int step = clk.step;
         if (DO_G2P) record = record_mode >= 0 ? record_mode != 0 : (step % c.update_every) == 0;
 
-        if (tid < a.tx) {
+        // point runs of the columns of the tile's occupied box
+        if (tid < ncol) {
             const int col = ti.col0 + tid;
             long long s0 = 0, s1 = 0;
             if (col < c.ncols) {
                 const long long base = S.cell_off + (long long)col * gridY;
                 s0 = a.cell_start[base + ti.row0];
-                s1 = a.cell_start[base + min(ti.row0 + a.ty, gridY)];
+                s1 = a.cell_start[base + min(ti.row0 + nrow, gridY)];
             }
             run_start[tid] = s0;
             run_pre[tid + 1] = (int)(s1 - s0);
@@ -229,10 +235,13 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
             constexpr int kU = 1;
             for (int i0 = tid; i0 < WN; i0 += kU * kBlock) {
                 float g6[kU][6];
+                int qc[kU], qr[kU];
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
                     const int i = i0 + u * kBlock;
-                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    qc[u] = W.col_of(i);
+                    qr[u] = i - qc[u] * win_h;
+                    const int col = wc0 + qc[u], row = wr0 + qr[u];
                     const bool in = i < WN && col >= 0 && col < c.ncols && row >= 0 && row < gridY;
                     const long long gi = (long long)col * gridY + row;
 #pragma unroll
@@ -242,7 +251,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                 for (int u = 0; u < kU; u++) {
                     const int i = i0 + u * kBlock;
                     if (i >= WN) break;
-                    const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                    const int col = wc0 + qc[u], row = wr0 + qr[u];
                     float vx = 0, vy = 0;
                     if (g6[u][kLineMHi] != 0) {
                         const double mass = (double)g6[u][kLineMHi] + (double)g6[u][kLineMLo];
@@ -253,35 +262,37 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                         vx = (float)dvx;
                         vy = (float)dvy;
                     }
-                    vw[2 * i] = vx;
-                    vw[2 * i + 1] = vy;
+                    const int li = qc[u] * WS + qr[u];
+                    vw[2 * li] = vx;
+                    vw[2 * li + 1] = vy;
                 }
             }
         }
         for (int i = tid; i < WN; i += kBlock) {
+            const int qc = W.col_of(i), qr = i - qc * win_h, li = qc * WS + qr;
             if (DO_G2P && !FROM_M) {
-                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                const int col = wc0 + qc, row = wr0 + qr;
                 float vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
                     const long long gi = (long long)col * gridY + row;
                     vx = gr[kLinePxHi * gp + gi];
                     vy = gr[kLinePyHi * gp + gi];
                 }
-                vw[2 * i] = vx;
-                vw[2 * i + 1] = vy;
+                vw[2 * li] = vx;
+                vw[2 * li + 1] = vy;
             }
             if (DO_P2G) {
 #pragma unroll
-                for (int q = 0; q < 3; q++) acc[3 * i + q] = acc[3 * WN + 3 * i + q] = 0;
+                for (int q = 0; q < 3; q++) acc[3 * li + q] = acc[3 * WNmax + 3 * li + q] = 0;
             }
         }
         __syncthreads();
         if (tid == 0) {
             run_pre[0] = 0;
-            for (int k = 0; k < a.tx; k++) run_pre[k + 1] += run_pre[k];
+            for (int k = 0; k < ncol; k++) run_pre[k + 1] += run_pre[k];
         }
         __syncthreads();
-        const int total = run_pre[a.tx];
+        const int total = run_pre[ncol];
 
         unsigned err = 0;
         int cidx = 0;
@@ -344,7 +355,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                 unsigned e;
                 // a point whose G2P stencil left the window goes to the
                 // deferred list, handled by the launch's last workgroup
-                if (!g2p_point(c, cf, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, FROM_M, xq, yq, v, C, D, Jm1, ud,
+                if (!g2p_point(c, cf, gr, gp, nullptr, vw, wc0, wr0, win_w, win_h, WS, FROM_M, xq, yq, v, C, D, Jm1, ud,
                                pp, qq, e, (unsigned)pt, (unsigned)step, terms)) {
                     if (e) {
                         err = max(err, e);
@@ -373,9 +384,9 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                 }
                 const int lx = st.bxc - wc0, ly = st.by - wr0;
                 if (lx >= 0 && lx + 2 < win_w && ly >= 0 && ly + 2 < win_h) {
-                    lds_d *const p0 = acc_lane + 3 * (lx * win_h + ly);
+                    lds_d *const p0 = acc_lane + 3 * (lx * WS + ly);
                     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
-                        lds_d *const p = p0 + 3 * (i * win_h + j);
+                        lds_d *const p = p0 + 3 * (i * WS + j);
                         lds_add(p, dm);
                         lds_add(p + 1, dpx);
                         lds_add(p + 2, dpy);
@@ -384,7 +395,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                     // the stencil leaves the window (a point that drifted since the sort)
                     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
                         if (lx + i >= 0 && lx + i < win_w && ly + j >= 0 && ly + j < win_h) {
-                            lds_d *const p = acc_lane + 3 * ((lx + i) * win_h + ly + j);
+                            lds_d *const p = acc_lane + 3 * ((lx + i) * WS + ly + j);
                             lds_add(p, dm);
                             lds_add(p + 1, dpx);
                             lds_add(p + 2, dpy);
@@ -403,14 +414,17 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
         if (DO_P2G) {
             __syncthreads();
             for (int i = tid; i < WN; i += kBlock) {
-                const double m = acc[3 * i] + acc[3 * WN + 3 * i];
+                const int qc = W.col_of(i), qr = i - qc * win_h, li = qc * WS + qr;
+                const double m = acc[3 * li] + acc[3 * WNmax + 3 * li];
                 if (m == 0) continue;
-                const int col = wc0 + i / win_h, row = wr0 + i % win_h;
+                const int col = wc0 + qc, row = wr0 + qr;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
                 twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, m);
-                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, acc[3 * i + 1] + acc[3 * WN + 3 * i + 1]);
-                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, acc[3 * i + 2] + acc[3 * WN + 3 * i + 2]);
+                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi,
+                             acc[3 * li + 1] + acc[3 * WNmax + 3 * li + 1]);
+                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi,
+                             acc[3 * li + 2] + acc[3 * WNmax + 3 * li + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }

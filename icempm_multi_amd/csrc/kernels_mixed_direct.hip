@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     load_state(R, pt, xq, yq, D, Jm1);
     unsigned e;
     StressTerms terms;
-    if (!g2p_point(c, cf, (const glb_f *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, nullptr, 0, 0, 0, 0,
+    if (!g2p_point(c, cf, (const glb_f *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, nullptr, 0, 0, 0, 0, 0,
                    false, xq, yq, v, C, D, Jm1, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step, terms)) {
         atomicMax(&a.err[scene], e);
         return;

@@ -105,10 +105,38 @@ struct SceneClock {
 };
 
 // A tile of TX x TY cells of one scene: the unit of the binned P2G / fused G2P.
+// (col0, row0) and dims (ncols | nrows << 16) give the bounding box of the
+// tile's cells that held points at the last sort, inside the TX x TY block:
+// the points are those of the box's cells, and the tile's LDS window is the
+// box widened by the drift margin and the stencil's one node on every side.
+// Tiles of a sparse scene (floe edges, gaps) thus stage fewer empty nodes.
 struct TileItem {
     int scene;
     int col0, row0;
-    int npts;
+    int dims;
+};
+
+// Node window of a tile in LDS: origin (c0, r0) in grid nodes, w x h nodes,
+// node i at column i / h and row i % h.
+struct TileWindow {
+    int c0, r0, w, h;
+    float inv_h;   // 1/h, rounded
+
+    __device__ __forceinline__ TileWindow(const TileItem &ti, int margin)
+    {
+        c0 = ti.col0 - 1 - margin;
+        r0 = ti.row0 - 1 - margin;
+        w = (ti.dims & 0xffff) + 2 + 2 * margin;
+        h = (ti.dims >> 16) + 2 + 2 * margin;
+        inv_h = 1.0f / (float)h;
+    }
+    // i / h for 0 <= i < w*h: (i + 1/2)/h is at least 1/(2h) away from an
+    // integer, and the two roundings of the float product err by less than
+    // (i + 1/2) * 2^-23.  Windows fit the 64 KiB of LDS a workgroup may hold
+    // (at least 56 B per node: i < 1171, h < 391), so the error, under
+    // 1.4e-4, stays below 1/(2h) > 1.2e-3 and the truncation is exact.
+    __device__ __forceinline__ int col_of(int i) const { return (int)(((float)i + 0.5f) * inv_h); }
+    __device__ __forceinline__ int nodes() const { return w * h; }
 };
 
 // A run of up to 64 node segments of one scene (up to 4096 nodes): the unit
@@ -137,7 +165,7 @@ struct StepArgs {
     int nscenes;
     unsigned *err;            // per-scene error byte (max over points)
     int tx, ty, margin;       // tile shape and drift margin of the LDS windows
-    int win_w, win_h;         // tx+2+2*margin, ty+2+2*margin
+    int win_w, win_h;         // tx+2+2*margin, ty+2+2*margin: the largest tile window (LDS size)
     int lane_spread;          // lane-to-point map of the tile kernels: 0 identity, 1 over 16 cells, 2 over 32 cells
     int dyn_tiles;            // 1: tiles pulled from per-XCD atomic queues, 0: static round robin
     int *tile_queue;          // 8 queue counters + 1 finished-block counter, zero between launches

@@ -57,7 +57,8 @@ __device__ __forceinline__ int xcd_block()
 // (375-376) and runs the constitutive branch (378-391).  Returns false (and
 // sets the error byte) if the stencil leaves the grid.
 //
-// vw: the LDS window of node velocities (null: every node from global memory).
+// vw: the LDS window of node velocities (null: every node from global memory),
+// win_w x win_h nodes from grid node (wc0, wr0), node (i, j) at vw[2*(i*win_s + j)].
 // vclk: null if grid g holds node velocities (a grid update ran); otherwise g
 // holds momenta and a node read from global memory gets its velocity formed
 // here with that clock (node_velocity, no force).
@@ -67,7 +68,7 @@ __device__ __forceinline__ int xcd_block()
 template <class Tab>
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, Tab ftab, const glb_d *g, long long gpitch,
                                           const SceneClock *vclk, const lds_d *vw, int wc0, int wr0, int win_w,
-                                          int win_h, bool defer_outside, double &x, double &y, double v[2],
+                                          int win_h, int win_s, bool defer_outside, double &x, double &y, double v[2],
                                           double B[4], double F[4], double &Jp, long long &ud, double &p_out,
                                           double &q_out, unsigned &e, StressTerms &st)
 {
@@ -110,7 +111,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, Tab ftab, const 
             double ax = 0, ay = 0, bx = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const int li = (lx + i) * win_h + ly + j;
+                const int li = (lx + i) * win_s + ly + j;
                 const double nvx = vw[2 * li], nvy = vw[2 * li + 1];   // one 128-bit LDS read
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;

@@ -129,14 +129,15 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
     }
 }
 
-// G2P of one point from an LDS window of node velocities, or from global
-// memory if the stencil left the window: the velocity lines, or (vclk not
-// null) node velocities formed from the momenta with that clock.
+// G2P of one point from an LDS window of node velocities (win_w x win_h nodes
+// from grid node (wc0, wr0), node (i, j) at vw[2*(i*win_s + j)]), or from
+// global memory if the stencil left the window: the velocity lines, or (vclk
+// not null) node velocities formed from the momenta with that clock.
 // defer_outside: if the stencil leaves the window, return false with e = 0
 // and change nothing (the caller handles the point later).
 __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_f *g, long long gp,
                                           const SceneClock *vclk, const lds_f *vw, int wc0, int wr0, int win_w,
-                                          int win_h, bool defer_outside, unsigned &xq, unsigned &yq, float v[2],
+                                          int win_h, int win_s, bool defer_outside, unsigned &xq, unsigned &yq, float v[2],
                                           float C[4], float D[4], float &Jm1, unsigned &ud, float &p_out,
                                           float &q_out, unsigned &e, unsigned point, unsigned step, StressTerms &st)
 {
@@ -176,7 +177,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
             float ax = 0, ay = 0, bx_ = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const int li = (lx + i) * win_h + ly + j;
+                const int li = (lx + i) * win_s + ly + j;
                 const float nvx = vw[2 * li], nvy = vw[2 * li + 1];   // (vx, vy) interleaved: one 64-bit LDS read
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;
