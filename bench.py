@@ -48,8 +48,8 @@ METRIC = "M particle-steps/s + HBM GB/s fraction, mixed vs FP64 at ≤1e-4 rel p
 # node it reads the momenta of the last P2G (24 B) and forms the velocity
 # while staging (the grid update folded in), accumulates mass and momentum
 # into the next buffer (24 B) and clears the buffer two substeps back (24 B).
-# The mixed form keeps 4-byte point fields (32 + 24 B per point) and
-# compensated (hi, lo) FP32 pairs on the grid (also 24 B per node and pass).
+# The mixed form keeps 4-byte point fields (32 + 24 B per point) and the same
+# FP64 grid (24 B per node and pass).
 BYTES = {
     "fp64": {"fused_point": 64 + 48, "fused_node": 24 + 24 + 24},
     "mixed": {"fused_point": 32 + 24, "fused_node": 24 + 24 + 24},

@@ -36,6 +36,9 @@ extern const int kNumTileCandidates;
 // launch of substep k reads buffer k (momenta), accumulates into k+1 and
 // clears k+2 (mod 3).
 constexpr int kGridBuffers = 3;
+// FP64 lines per grid buffer: m, px, py (the node velocity in lines 1-2 after
+// a separate grid update), in both precision forms.
+constexpr int kGridLines = 3;
 
 struct SceneHost {
     icempm_simparams prms{};

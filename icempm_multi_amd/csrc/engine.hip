@@ -391,11 +391,12 @@ int Engine::commit()
         HIPCHK(hipMemsetAsync(d_defer_ + 2 * n_pts_space_, 0, sizeof(int) * 64, stream_));
     }
 
-    // grids: [buffer 0 of all scenes][buffer 1 ...][buffer 2 ...], 24 bytes per node each
+    // grids: [buffer 0 of all scenes][buffer 1 ...][buffer 2 ...], 3 FP64 lines
+    // (m, px, py) per scene in both precision forms, 24 bytes per node
     if (grid_mem_) { hipFree(grid_mem_); grid_mem_ = nullptr; }
     long long gelems = 0, cells = 0, forces = 0;
     std::vector<long long> goff(nscenes()), coff(nscenes()), foff(nscenes());
-    const long long lines = precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3;
+    const long long lines = kGridLines;
     for (int i = 0; i < nscenes(); i++) {
         const icempm_simparams &P = scenes_[i].prms;
         const long long ncols = P.GridXTotal + 2LL * P.GridHaloSize;
@@ -410,7 +411,7 @@ int Engine::commit()
     }
     // cell keys, bin counts and cell starts are 32-bit
     if (cells >= (1LL << 31) - 1) return fail(ICEMPM_ERR_CAPACITY, "grid cells of all scenes exceed 2^31");
-    const size_t buf_bytes = (size_t)gelems * eb;
+    const size_t buf_bytes = (size_t)gelems * sizeof(double);
     HIPCHK(hipMalloc(&grid_mem_, kGridBuffers * buf_bytes));
     HIPCHK(hipMemsetAsync(grid_mem_, 0, kGridBuffers * buf_bytes, stream_));
     // segment flags (grid_elems is a multiple of the segment size); each scene's
@@ -427,7 +428,7 @@ int Engine::commit()
     HIPCHK(hipMalloc(&ind_force_mem_, sizeof(double) * std::max<long long>(forces, 1)));
     HIPCHK(hipMemsetAsync(ind_force_mem_, 0, sizeof(double) * std::max<long long>(forces, 1), stream_));
     for (int i = 0; i < nscenes(); i++) {
-        for (int b = 0; b < kGridBuffers; b++) scenes_[i].grid[b] = (char *)grid_mem_ + b * buf_bytes + goff[i] * eb;
+        for (int b = 0; b < kGridBuffers; b++) scenes_[i].grid[b] = (char *)grid_mem_ + b * buf_bytes + goff[i] * sizeof(double);
         scenes_[i].cells_off = coff[i];
         scenes_[i].ind_force = ind_force_mem_ + foff[i];
         for (int b = 0; b < kGridBuffers; b++) scenes_[i].clean[b] = true;
@@ -747,8 +748,7 @@ int Engine::get_kernel_stats(double *ms, long long *count, int n)
 int Engine::zero_grid(int scene, int buf)
 {
     const SceneHost &s = scenes_[scene];
-    HIPCHK(launch_fill_zero(s.grid[buf], (size_t)s.grid_elems * (precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3) *
-                                             elem_bytes(), stream_));
+    HIPCHK(launch_fill_zero(s.grid[buf], (size_t)s.grid_elems * kGridLines * sizeof(double), stream_));
     HIPCHK(launch_fill_zero(d_dirty_ + buf * n_segs_ + s.seg_off, sizeof(int) * (s.grid_elems >> kSegShift),
                             stream_));
     scenes_[scene].clean[buf] = true;
@@ -828,7 +828,7 @@ int Engine::update_nodes(int scene, double t, double ind_x, double ind_y)
     a.chunks = first;
     a.n_chunks = nchunks;
     timed_begin(true);
-    HIPCHK(LAUNCH_PREC(launch_update, a, 0, 0, 0, update_blocks(nchunks), stream_));
+    HIPCHK(fp64::launch_update(a, 0, 0, 0, update_blocks(nchunks), stream_));
     timed_end(true, kClsUpdate);
     HIPCHK(hipStreamSynchronize(stream_));   // ck lives on this stack frame
     return ICEMPM_OK;
@@ -914,20 +914,9 @@ int Engine::get_grid(int scene, int buf, double *out, long long n_nodes)
     const long long nodes = (long long)(s.prms.GridXTotal + 2 * s.prms.GridHaloSize) * s.prms.GridY;
     if (n_nodes != nodes) return fail(ICEMPM_ERR_ARGUMENT, "get_grid: node count mismatch");
     HIPCHK(hipStreamSynchronize(stream_));
-    if (precision_ == ICEMPM_PRECISION_FP64) {
-        for (int line = 0; line < 3; line++)
-            HIPCHK(hipMemcpy(out + line * nodes, (double *)s.grid[buf] + line * s.grid_elems, sizeof(double) * nodes,
-                             hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> hi(nodes), lo(nodes);
-        for (int line = 0; line < 3; line++) {
-            HIPCHK(hipMemcpy(hi.data(), (float *)s.grid[buf] + (2 * line) * s.grid_elems, sizeof(float) * nodes,
-                             hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(lo.data(), (float *)s.grid[buf] + (2 * line + 1) * s.grid_elems, sizeof(float) * nodes,
-                             hipMemcpyDeviceToHost));
-            for (long long i = 0; i < nodes; i++) out[line * nodes + i] = (double)hi[i] + (double)lo[i];
-        }
-    }
+    for (int line = 0; line < kGridLines; line++)
+        HIPCHK(hipMemcpy(out + line * nodes, (double *)s.grid[buf] + line * s.grid_elems, sizeof(double) * nodes,
+                         hipMemcpyDeviceToHost));
     return ICEMPM_OK;
 }
 
@@ -1044,9 +1033,7 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++) {
                 const SceneHost &s = scenes_[i];
-                HIPCHK(launch_fill_zero(s.grid[0], (size_t)s.grid_elems *
-                                                       (precision_ == ICEMPM_PRECISION_MIXED ? 6 : 3) * elem_bytes(),
-                                        stream_));
+                HIPCHK(launch_fill_zero(s.grid[0], (size_t)s.grid_elems * kGridLines * sizeof(double), stream_));
             }
             HIPCHK(launch_fill_zero(d_dirty_, sizeof(int) * n_segs_, stream_));
             timed_end(timed, kClsZero);
@@ -1055,7 +1042,7 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
                 HIPCHK(LAUNCH_PREC(launch_p2g_direct, au, i, scenes_[i].n_pts, 0, stream_));
             timed_end(timed, kClsP2G);
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_update, au, 0, 0, 1, nb_upd, stream_));
+            HIPCHK(fp64::launch_update(au, 0, 0, 1, nb_upd, stream_));
             timed_end(timed, kClsUpdate);
             timed_begin(timed);
             for (int i = 0; i < nscenes(); i++)
@@ -1068,7 +1055,7 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
             HIPCHK(LAUNCH_PREC(launch_tiles, au, kTileP2G, b, b, b, -1, tile_blocks(kTileP2G), stream_));
             timed_end(timed, kClsP2G);
             timed_begin(timed);
-            HIPCHK(LAUNCH_PREC(launch_update, au, b, 1, 1, nb_upd, stream_));
+            HIPCHK(fp64::launch_update(au, b, 1, 1, nb_upd, stream_));
             timed_end(timed, kClsUpdate);
             timed_begin(timed);
             HIPCHK(LAUNCH_PREC(launch_tiles, an, kTileG2P, b, b, b, -1, tile_blocks(kTileG2P), stream_));

@@ -27,6 +27,8 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbu
                         hipStream_t s);
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
+// Grid update of both precision forms: their grids share one layout (3 FP64
+// lines per buffer) and the update works in FP64.
 hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 // Workgroups of the tile kernel for this mode and window that one CU holds at
@@ -40,7 +42,6 @@ hipError_t launch_tiles(const StepArgs &a, int mode, int rbuf, int wbuf, int zbu
                         hipStream_t s);
 hipError_t launch_p2g_direct(const StepArgs &a, int scene, int n_pts, int wbuf, hipStream_t s);
 hipError_t launch_g2p_direct(const StepArgs &a, int scene, int n_pts, int rbuf, int record_pq, hipStream_t s);
-hipError_t launch_update(const StepArgs &a, int buf, int zero_other, int advance_clock, int n_blocks, hipStream_t s);
 size_t tile_lds_bytes(const StepArgs &a, int mode);
 // Workgroups of the tile kernel for this mode and window that one CU holds at
 // once (registers and LDS); 0 if the runtime cannot tell.

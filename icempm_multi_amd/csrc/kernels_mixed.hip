@@ -1,14 +1,12 @@
-// kernels_mixed.hip -- mixed-precision binned tile kernels (the grid update
-// is in kernels_mixed_update.hip).
+// kernels_mixed.hip -- mixed-precision binned tile kernels (the separate grid
+// update is the FP64 form's, kernels_fp64_update.hip).
 //
 // Same pipeline as kernels_fp64.hip, with FP32 point state (points read and
-// written: 32 + 56 bytes per substep instead of 64 + 112) and compensated grid
-// sums: within a tile the FP32 contributions are summed in FP64 LDS
-// accumulators (ds_add_f64, exact for the few hundred terms a node gets per
-// tile); across tiles each node value is split into an FP32 (hi, lo) pair and
-// added with an atomic two-sum, so the stored hi + lo keeps ~48 bits.  The
-// grid update forms hi + lo in FP64, divides momentum by mass in FP64 and
-// stores the velocity in FP32.
+// written: 32 + 56 bytes per substep instead of 64 + 112) and the FP64 grid:
+// within a tile the FP32 contributions are summed in FP64 LDS accumulators
+// (ds_add_f64), and each node's sums are added into the grid with FP64
+// atomics.  Node velocities are formed in FP64 (momentum over mass) and kept
+// in FP32 in the tile's LDS window.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -18,7 +16,7 @@ namespace icempm {
 namespace mixed {
 
 // Parts of the grid update that the tile kernels take over (kTileFromMomenta),
-// as in the FP64 form, on the compensated FP32 grid (6 lines per buffer).
+// as in the FP64 form (the grid is the same: 3 FP64 lines per buffer).
 __device__ __forceinline__ void clear_flagged(const StepArgs &a, int zb)
 {
     const int lane = threadIdx.x & 63;
@@ -26,7 +24,7 @@ __device__ __forceinline__ void clear_flagged(const StepArgs &a, int zb)
     for (int r = wave; r < a.n_chunks; r += n_waves) {
         const NodeChunk run = uniform_copy(a.chunks[r]);
         const SceneDev &S = a.scenes[run.scene];
-        glb_f *g = (glb_f *)(a.grid_mem + S.grid_off[zb]);
+        glb_d *g = (glb_d *)(a.grid_mem + S.grid_off[zb]);
         const long long gp = S.gpitch;
         int *fl = a.dirty + zb * a.dirty_stride + S.seg_off + run.seg0;
         const int f = lane < run.nseg ? fl[lane] : 0;
@@ -37,7 +35,7 @@ __device__ __forceinline__ void clear_flagged(const StepArgs &a, int zb)
             mb &= mb - 1;
             const long long idx = node0 + ((long long)sg << kSegShift) + lane;
 #pragma unroll
-            for (int l = 0; l < 6; l++) g[l * gp + idx] = 0;
+            for (int l = 0; l < 3; l++) g[l * gp + idx] = 0;
         }
         if (f) fl[lane] = 0;
     }
@@ -59,25 +57,22 @@ __device__ __forceinline__ void indenter_force_nodes(const StepArgs &a, int sc, 
     if (gx1 < gx0 || gy1 < gy0) return;
     const int ny = gy1 - gy0 + 1;
     const long long nn = (long long)(gx1 - gx0 + 1) * ny;
-    const glb_f *g = (const glb_f *)(a.grid_mem + S.grid_off[rb]);
+    const glb_d *g = (const glb_d *)(a.grid_mem + S.grid_off[rb]);
     const long long gp = S.gpitch;
     double *force = a.force_mem + S.force_off;
     for (long long k = threadIdx.x; k < nn; k += kBlock) {
         const int gix = gx0 + (int)(k / ny), giy = gy0 + (int)(k % ny);
         const long long idx = (long long)(gix - c.xoff + c.halo) * c.gridY + giy;
-        const float mh = g[kLineMHi * gp + idx];
-        if (mh == 0) continue;
-        const double mass = (double)mh + (double)g[kLineMLo * gp + idx];
-        const double px = (double)g[kLinePxHi * gp + idx] + (double)g[kLinePxLo * gp + idx];
-        const double py = (double)g[kLinePyHi * gp + idx] + (double)g[kLinePyLo * gp + idx];
+        const double mass = g[kLineM * gp + idx];
+        if (mass == 0) continue;
+        const double px = g[kLinePx * gp + idx], py = g[kLinePy * gp + idx];
         double vx, vy;
         fp64::node_velocity(c, clk, force, gix, giy, mass, px, py, vx, vy);
     }
 }
 
 // Entry k of the deferred-point list, as in the FP64 form (G2P with node
-// velocities formed from the global momenta; P2G with compensated global
-// atomics).
+// velocities formed from the global momenta; P2G with FP64 global atomics).
 __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int rb, int wb, int k)
 {
     const PtRef R{(unsigned *)a.pts, a.pitch};
@@ -94,7 +89,7 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
     unsigned e;
     StressTerms terms;
     terms.valid = false;
-    const glb_f *gr = (const glb_f *)(a.grid_mem + S.grid_off[rb]);
+    const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rb]);
     if (!g2p_point(c, cf, gr, S.gpitch, &clk, nullptr, 0, 0, 0, 0, 0, false, xq, yq, v, C, D, Jm1, ud, pp, qq, e, pt,
                    (unsigned)clk.step, terms)) {
         atomicMax(&a.err[sc], e);
@@ -105,7 +100,7 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
         return;
     }
     store_state_fused(R, pt, xq, yq, D, Jm1, ud, record, pp, qq);
-    glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wb]);
+    glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wb]);
     const long long gp = S.gpitch;
     const int gridY = c.gridY;
     P2GStencil st;
@@ -116,9 +111,7 @@ __device__ __forceinline__ void deferred_point(const StepArgs &a, bool p2g, int 
     }
     st.nodes(cf, [&](int i, int j, double dm, double dpx, double dpy) {
         const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
-        twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
-        twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
-        twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy);
+        grid_add(gw, gp, gi, dm, dpx, dpy);
         mark_dirty(a, S, wb, gi);
     });
 }
@@ -204,8 +197,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
         const TileWindow W(ti, a.margin);
         const int wc0 = W.c0, wr0 = W.r0, win_w = W.w, win_h = W.h, WN = W.nodes();
         const int ncol = ti.dims & 0xffff, nrow = ti.dims >> 16;
-        const glb_f *gr = (const glb_f *)(a.grid_mem + S.grid_off[rbuf]);
-        glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
+        const glb_d *gr = (const glb_d *)(a.grid_mem + S.grid_off[rbuf]);
+        glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
         const long long gp = S.gpitch;
         int *const dflag = a.dirty + wbuf * a.dirty_stride + uniform_copy(S.seg_off);   // segment flags of wbuf
         // the substep's clock: advanced here from the current slot, or already
@@ -229,12 +222,12 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
             run_pre[tid + 1] = (int)(s1 - s0);
         }
         if (DO_G2P && FROM_M) {
-            // velocities formed from the compensated momenta, one window node
-            // per thread and pass (two at once spill a register in the point
-            // loop's kernel)
+            // velocities formed from the momenta, one window node per thread
+            // and pass (two at once spill a register in the point loop's
+            // kernel)
             constexpr int kU = 1;
             for (int i0 = tid; i0 < WN; i0 += kU * kBlock) {
-                float g6[kU][6];
+                double g3[kU][3];
                 int qc[kU], qr[kU];
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
@@ -245,7 +238,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                     const bool in = i < WN && col >= 0 && col < c.ncols && row >= 0 && row < gridY;
                     const long long gi = (long long)col * gridY + row;
 #pragma unroll
-                    for (int l = 0; l < 6; l++) g6[u][l] = in ? (float)gr[l * gp + gi] : 0.f;
+                    for (int l = 0; l < 3; l++) g3[u][l] = in ? (double)gr[l * gp + gi] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < kU; u++) {
@@ -253,12 +246,10 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                     if (i >= WN) break;
                     const int col = wc0 + qc[u], row = wr0 + qr[u];
                     float vx = 0, vy = 0;
-                    if (g6[u][kLineMHi] != 0) {
-                        const double mass = (double)g6[u][kLineMHi] + (double)g6[u][kLineMLo];
-                        const double px = (double)g6[u][kLinePxHi] + (double)g6[u][kLinePxLo];
-                        const double py = (double)g6[u][kLinePyHi] + (double)g6[u][kLinePyLo];
+                    if (g3[u][kLineM] != 0) {
                         double dvx, dvy;
-                        fp64::node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, mass, px, py, dvx, dvy);
+                        fp64::node_velocity(c, clk, nullptr, col - c.halo + c.xoff, row, g3[u][kLineM],
+                                            g3[u][kLinePx], g3[u][kLinePy], dvx, dvy);
                         vx = (float)dvx;
                         vy = (float)dvy;
                     }
@@ -275,8 +266,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                 float vx = 0, vy = 0;
                 if (col >= 0 && col < c.ncols && row >= 0 && row < gridY) {
                     const long long gi = (long long)col * gridY + row;
-                    vx = gr[kLinePxHi * gp + gi];
-                    vy = gr[kLinePyHi * gp + gi];
+                    vx = (float)gr[kLinePx * gp + gi];
+                    vy = (float)gr[kLinePy * gp + gi];
                 }
                 vw[2 * li] = vx;
                 vw[2 * li + 1] = vy;
@@ -401,9 +392,7 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                             lds_add(p + 2, dpy);
                         } else {
                             const long long gi = (long long)(st.bxc + i) * gridY + st.by + j;
-                            twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
-                            twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
-                            twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy);
+                            grid_add(gw, gp, gi, dm, dpx, dpy);
                             dflag[gi >> kSegShift] = 1;
                         }
                     });
@@ -420,11 +409,8 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
                 const int col = wc0 + qc, row = wr0 + qr;
                 if (col < 0 || col >= c.ncols || row < 0 || row >= gridY) continue;
                 const long long gi = (long long)col * gridY + row;
-                twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, m);
-                twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi,
-                             acc[3 * li + 1] + acc[3 * WNmax + 3 * li + 1]);
-                twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi,
-                             acc[3 * li + 2] + acc[3 * WNmax + 3 * li + 2]);
+                grid_add(gw, gp, gi, m, acc[3 * li + 1] + acc[3 * WNmax + 3 * li + 1],
+                         acc[3 * li + 2] + acc[3 * WNmax + 3 * li + 2]);
                 dflag[gi >> kSegShift] = 1;
             }
         }

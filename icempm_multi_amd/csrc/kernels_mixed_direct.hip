@@ -1,5 +1,5 @@
 // kernels_mixed_direct.hip -- unbinned mixed-precision P2G and G2P (one thread
-// per point; P2G adds every contribution with the atomic two-sum).  Separate
+// per point; P2G adds every contribution with FP64 global atomics).  Separate
 // translation unit for the same compiler reason as kernels_fp64_direct.hip.
 #include <hip/hip_runtime.h>
 
@@ -26,14 +26,12 @@ __global__ void __launch_bounds__(kBlock) k_p2g_direct(StepArgs a, int scene, in
     v[1] = R.f(kLineVelX + 1, pt);
 #pragma unroll
     for (int m = 0; m < 4; m++) C[m] = R.f(kLineBp00 + m, pt);
-    glb_f *gw = (glb_f *)(a.grid_mem + S.grid_off[wbuf]);
+    glb_d *gw = (glb_d *)(a.grid_mem + S.grid_off[wbuf]);
     const long long gp = S.gpitch;
     const int gridY = c.gridY;
     const unsigned e = p2g_point(c, cf, xq, yq, v, C, D, Jm1, ud, [&](int col, int row, double dm, double dpx, double dpy) {
         const long long gi = (long long)col * gridY + row;
-        twosum_add_d(gw + kLineMHi * gp + gi, gw + kLineMLo * gp + gi, dm);
-        twosum_add_d(gw + kLinePxHi * gp + gi, gw + kLinePxLo * gp + gi, dpx);
-        twosum_add_d(gw + kLinePyHi * gp + gi, gw + kLinePyLo * gp + gi, dpy);
+        grid_add(gw, gp, gi, dm, dpx, dpy);
         mark_dirty(a, S, wbuf, gi);
     });
     if (e) atomicMax(&a.err[scene], e);
@@ -55,7 +53,7 @@ __global__ void __launch_bounds__(kBlock) k_g2p_direct(StepArgs a, int scene, in
     load_state(R, pt, xq, yq, D, Jm1);
     unsigned e;
     StressTerms terms;
-    if (!g2p_point(c, cf, (const glb_f *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, nullptr, 0, 0, 0, 0, 0,
+    if (!g2p_point(c, cf, (const glb_d *)(a.grid_mem + S.grid_off[rbuf]), S.gpitch, nullptr, nullptr, 0, 0, 0, 0, 0,
                    false, xq, yq, v, C, D, Jm1, ud, pp, qq, e, (unsigned)pt, (unsigned)a.clocks[scene].step, terms)) {
         atomicMax(&a.err[scene], e);
         return;

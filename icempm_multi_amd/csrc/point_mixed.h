@@ -4,8 +4,9 @@
 // 3 Jp_inv - 1, 4-5 fixed-point x/y, 6-7 v, 8-11 D = F - I, 12-15 C (FP32).
 // Jp_inv and F are stored as their difference from 1 and I: a liquid point's
 // Jp_inv moves by about dt * div v per substep, a few FP32 ulps of 1.
-// Grid buffer: 6 FP32 lines (m, px, py as compensated hi/lo pairs); after the
-// grid update lines 2 and 4 hold the node velocity.
+// Grid buffer: 3 FP64 lines (m, px, py), as in the FP64 form: the P2G sums in
+// FP64 (LDS windows, then FP64 atomics into the grid); after the grid update
+// lines 1 and 2 hold the node velocity.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,35 +20,30 @@ namespace mixed {
 
 typedef __attribute__((address_space(3))) float lds_f;
 typedef __attribute__((address_space(3))) double lds_d;
-typedef __attribute__((address_space(1))) float glb_f;
+typedef __attribute__((address_space(1))) double glb_d;
 typedef __attribute__((address_space(1))) unsigned glb_u;
 
 constexpr int kBlock = 256;
 constexpr int kMaxTX = 32;
-constexpr int kLineMHi = 0, kLineMLo = 1, kLinePxHi = 2, kLinePxLo = 3, kLinePyHi = 4, kLinePyLo = 5;
+constexpr int kLineM = 0, kLinePx = 1, kLinePy = 2;   // grid lines
 
 __device__ __forceinline__ void lds_add(lds_d *p, double v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Compensated accumulation of v into the (hi, lo) pair of a node: the atomic
-// returns the old hi, so the rounding error of the stored hi + v is recovered
-// exactly (Knuth's two-sum) and added to lo.
-__device__ __forceinline__ void twosum_add(glb_f *hi, glb_f *lo, float v, float vlo)
+// Non-returning FP64 atomic add into the grid.
+__device__ __forceinline__ void glb_add(glb_d *p, double v)
 {
-    const float old = __hip_atomic_fetch_add(hi, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const float s = old + v;
-    const float bb = s - old;
-    const float err = (old - (s - bb)) + (v - bb);
-    const float e = err + vlo;
-    if (e != 0.f) __hip_atomic_fetch_add(lo, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void twosum_add_d(glb_f *hi, glb_f *lo, double v)
+// Adds one node's (m, px, py) into grid g (line pitch gp).
+__device__ __forceinline__ void grid_add(glb_d *g, long long gp, long long gi, double m, double px, double py)
 {
-    const float h = (float)v;
-    twosum_add(hi, lo, h, (float)(v - (double)h));
+    glb_add(g + kLineM * gp + gi, m);
+    glb_add(g + kLinePx * gp + gi, px);
+    glb_add(g + kLinePy * gp + gi, py);
 }
 
 // SoA access: uniform line base + 32-bit byte offset (saddr + voffset form).
@@ -135,7 +131,7 @@ __device__ __forceinline__ void constitutive(const SceneConsts &c, const ConstsF
 // not null) node velocities formed from the momenta with that clock.
 // defer_outside: if the stencil leaves the window, return false with e = 0
 // and change nothing (the caller handles the point later).
-__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_f *g, long long gp,
+__device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &cf, const glb_d *g, long long gp,
                                           const SceneClock *vclk, const lds_f *vw, int wc0, int wr0, int win_w,
                                           int win_h, int win_s, bool defer_outside, unsigned &xq, unsigned &yq, float v[2],
                                           float C[4], float D[4], float &Jm1, unsigned &ud, float &p_out,
@@ -187,29 +183,27 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
             column(i, ax, ay, bx_, by_);
         }
     } else if (vclk) {
-        // compensated momenta in global memory (the points a launch set
-        // aside, handled at its end): the loads of the 9 nodes issued
-        // together, then each node's velocity formed here
-        float gl[9][6];
+        // momenta in global memory (the points a launch set aside, handled
+        // at its end): the loads of the 9 nodes issued together, then each
+        // node's velocity formed here
+        double gl[9][3];
 #pragma unroll
         for (int n = 0; n < 9; n++) {
             const long long gi = (long long)(bxc + n / 3) * c.gridY + by + n % 3;
 #pragma unroll
-            for (int l = 0; l < 6; l++) gl[n][l] = g[l * gp + gi];
+            for (int l = 0; l < 3; l++) gl[n][l] = g[l * gp + gi];
         }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             float ax = 0, ay = 0, bx_ = 0, by_ = 0;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const float *q = gl[3 * i + j];
+                const double *q = gl[3 * i + j];
                 float nvx = 0, nvy = 0;
-                if (q[kLineMHi] != 0) {
+                if (q[kLineM] != 0) {
                     double dvx, dvy;
-                    fp64::node_velocity(c, *vclk, nullptr, bxc + i - c.halo + c.xoff, by + j,
-                                        (double)q[kLineMHi] + (double)q[kLineMLo],
-                                        (double)q[kLinePxHi] + (double)q[kLinePxLo],
-                                        (double)q[kLinePyHi] + (double)q[kLinePyLo], dvx, dvy);
+                    fp64::node_velocity(c, *vclk, nullptr, bxc + i - c.halo + c.xoff, by + j, q[kLineM],
+                                        q[kLinePx], q[kLinePy], dvx, dvy);
                     nvx = (float)dvx;
                     nvy = (float)dvy;
                 }
@@ -227,7 +221,7 @@ __device__ __forceinline__ bool g2p_point(const SceneConsts &c, const ConstsF &c
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const long long gi = (long long)(bxc + i) * c.gridY + by + j;
-                const float nvx = g[kLinePxHi * gp + gi], nvy = g[kLinePyHi * gp + gi];
+                const float nvx = (float)g[kLinePx * gp + gi], nvy = (float)g[kLinePy * gp + gi];
                 ax += wy[j] * nvx;
                 ay += wy[j] * nvy;
                 bx_ += wyd[j] * nvx;

@@ -99,7 +99,7 @@ typedef struct icempm_simparams {
 
 /* Precision forms */
 #define ICEMPM_PRECISION_FP64   0   /* FP64 state and arithmetic (the reference's form) */
-#define ICEMPM_PRECISION_MIXED  1   /* FP32 state, compensated grid sums, FP64 islands */
+#define ICEMPM_PRECISION_MIXED  1   /* FP32 state, FP64 grid sums, FP64 islands */
 
 /* Status codes */
 #define ICEMPM_OK               0
@@ -235,10 +235,10 @@ int icempm_partition_sort_points(icempm_partition *p);
 int icempm_partition_get_bins(icempm_partition *p, int32_t *keys, int32_t *perm, int n,
                               int32_t *cell_counts, int n_cells);
 
-/* Copy of grid buffer `buf` (0 or 1) as 3 lines of n_nodes doubles (mass,
+/* Copy of grid buffer `buf` (0, 1 or 2) as 3 lines of n_nodes doubles (mass,
  * momentum or velocity x, y), n_nodes = GridY*(GridXTotal + 2*GridHaloSize),
- * node (gx, gy) at (gx+GridHaloSize)*GridY + gy.  For inspection and tests; the
- * mixed form returns hi+lo of its compensated pairs.  No reference counterpart
+ * node (gx, gy) at (gx+GridHaloSize)*GridY + gy; both precision forms keep
+ * the grid in FP64.  For inspection and tests.  No reference counterpart
  * (the reference keeps the grid internal). */
 int icempm_partition_get_grid(icempm_partition *p, int buf, double *out, long long n_nodes);
 

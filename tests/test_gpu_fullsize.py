@@ -66,7 +66,7 @@ def test_sheet_4m_binning_mass_and_state(sheet, precision):
     part.p2g()
     g = part.get_grid(0, O.n_nodes(P))
     # the mixed form carries the particle mass in FP32 (0.6125 kg is not exact
-    # in binary); the compensated grid sums then conserve that mass
+    # in binary); the FP64 grid sums then conserve that mass
     m = P.ParticleMass if precision == im.PRECISION_FP64 else float(np.float32(P.ParticleMass))
     M = n * m
     assert abs(g[0].sum() - M) <= 1e-9 * M

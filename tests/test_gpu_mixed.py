@@ -1,9 +1,9 @@
-"""Mixed-precision form (FP32 state, fixed-point positions, compensated grid
-sums, FP64 islands) vs the FP64 oracle, through the C-ABI.
+"""Mixed-precision form (FP32 state, fixed-point positions, FP64 grid sums,
+FP64 islands) vs the FP64 oracle, through the C-ABI.
 
 Bars: binning bit-exact on the stored (fixed-point) positions; relative
 position error ||X - X_ref|| / ||X_ref|| <= 1e-4 after the block workload's
-steps; mass conserved to 1e-9 by the compensated sums.  The other per-field
+steps; mass conserved to 1e-9 by the FP64 grid sums.  The other per-field
 errors are reported (printed) with loose bounds that catch gross mistakes.
 """
 import numpy as np
@@ -90,7 +90,7 @@ def test_mixed_binning_bit_exact():
     assert np.array_equal(counts, np.bincount(keys[order] + P.GridHaloSize * P.GridY, minlength=ncells))
 
 
-def test_compensated_grid_mass():
+def test_grid_mass_conserved():
     P, h, n = scenes.block_scene(seed=0x1CE0901)
     part = make(P, h, n, im.PRECISION_MIXED)
     part.sort_points()
