@@ -222,10 +222,9 @@ k_tiles(StepArgs a, int rbuf, int wbuf, int zbuf, int record_mode)
             run_pre[tid + 1] = (int)(s1 - s0);
         }
         if (DO_G2P && FROM_M) {
-            // velocities formed from the momenta, one window node per thread
-            // and pass (two at once spill a register in the point loop's
-            // kernel)
-            constexpr int kU = 1;
+            // velocities formed from the momenta: four window nodes per
+            // thread and pass, the loads of all four issued before any is used
+            constexpr int kU = 4;
             for (int i0 = tid; i0 < WN; i0 += kU * kBlock) {
                 double g3[kU][3];
                 int qc[kU], qr[kU];
