@@ -82,7 +82,7 @@ struct SceneDev {
     int n_pts;              // points of the scene (after the last compaction)
     int cap_pts;            // capacity of the scene's point range
     long long grid_off[3];  // byte offsets of the three accumulation grids from StepArgs::grid_mem
-                            // (3 lines of gpitch elements, or 6 for mixed)
+                            // (3 FP64 lines of gpitch elements, both forms)
     long long gpitch;       // elements per grid line (>= ncols*gridY)
     long long cell_off;     // first entry of the scene in the cell_start array
     int ncells;             // ncols*gridY
