@@ -14,8 +14,9 @@ Prints one JSON line:
   replays, every graph captured before the timed region);
 - sort_ms_per_frame: the device re-sort of a frame boundary, timed on its own
   (inside the timed region only when it spans a frame boundary);
-- roofline: the fused G2P+P2G kernel's average launch time from hipEvents around
-  every launch of a separate 64-substep call, against its compulsory HBM bytes;
+- roofline: the substep kernel's average launch time from hipEvents around the
+  back-to-back substep launches of a separate 64-substep call (one pair per
+  graph chunk), against its compulsory HBM bytes;
 - mixed: the same for the mixed-precision form, with its accuracy against
   FP64 after 1000 substeps;
 - batch: the 64-scene workload, mixed batched, FP64 batched and the plain FP64
@@ -413,7 +414,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=500, help="timed substeps of the batch object")
     ap.add_argument("--accuracy-steps", type=int, default=1000, help="substeps of the mixed vs FP64 comparisons")
     ap.add_argument("--timing-substeps", type=int, default=64,
-                    help="substeps of the pass with hipEvents around every launch (kernel times, roofline)")
+                    help="substeps of the pass with hipEvents around the launches (kernel times, roofline)")
     ap.add_argument("--calib-copy", action="store_true", help="run a 1 GiB device copy first (counter calibration)")
     ap.add_argument("--tile", default=None, help="TXxTY[xMARGIN] tile shape instead of the autotuner's choice")
     ap.add_argument("--workload", choices=["sheet", "floes"], default="sheet")
@@ -479,9 +480,11 @@ def main():
             part.sort_points()
             sort_s.append(time.perf_counter() - ts)
         sort_ms = 1e3 * float(np.median(sort_s))
-        # kernel times: one more call with hipEvents around every launch
+        # kernel times: one more call with hipEvents, one pair around each run
+        # of back-to-back substep launches (and around the call's first P2G and
+        # last substep)
         part.reset_indenter_force_accumulator()          # also resets the kernel statistics
-        part.step(args.timing_substeps, flags | im.STEP_TIMINGS)
+        part.step(args.timing_substeps, flags | im.STEP_TIMINGS | im.STEP_SPAN)
         stats = part.kernel_stats()
         log(f"[{precision_name}] kernel stats (ms, launches):", {k: (round(v[0], 3), v[1]) for k, v in stats.items()})
         part.reset_grid()
@@ -530,8 +533,9 @@ def main():
         "roofline": primary["roofline"],
         "kernel_ms": primary["kernel_ms"],
         "timing_pass": f"kernel_ms and roofline.avg_launch_us: a {args.timing_substeps}-substep call after the timed "
-                       "region with hipEvents around every launch" +
-                       (" (recorded inside the replayed graphs)" if args.mode == "graph" else ""),
+                       "region with hipEvents, one pair around each run of back-to-back substep launches "
+                       + ("(each replayed graph chunk's, recorded inside it)" if args.mode == "graph"
+                          else "(the call's, direct launches)") + " and one around its first P2G and last substep",
     }
     if args.precision == "fp64" and not args.no_mixed:
         mx = timed("mixed")

@@ -8,7 +8,7 @@ on the CPU: without the built library every call raises.
 """
 from .params import (SimParams, FIELD_NAMES, N_PTS_ARRAYS, PRECISION_FP64, PRECISION_MIXED, STATUS_CRUSHED,
                      STATUS_DISABLED, STATUS_LIQUID, STEP_DIRECT_P2G, STEP_GRAPH, STEP_RESORT, STEP_TIMINGS,
-                     STEP_SAMPLED, STEP_UNFUSED, OK, ERR_ARGUMENT, ERR_CAPACITY, ERR_DEVICE, ERR_HIP, ERR_NOMEM,
+                     STEP_SAMPLED, STEP_SPAN, STEP_UNFUSED, OK, ERR_ARGUMENT, ERR_CAPACITY, ERR_DEVICE, ERR_HIP, ERR_NOMEM,
                      ERR_STATE)
 from .capi import (Batch, IcempmError, Partition, EXPORTED_SYMBOLS, LIB_PATH, lib, params_derive,
                    params_parse_json, params_reset, prepare_raw_points)
@@ -16,7 +16,7 @@ from .capi import (Batch, IcempmError, Partition, EXPORTED_SYMBOLS, LIB_PATH, li
 __all__ = [
     "SimParams", "FIELD_NAMES", "N_PTS_ARRAYS", "PRECISION_FP64", "PRECISION_MIXED", "STATUS_CRUSHED",
     "STATUS_DISABLED", "STATUS_LIQUID", "STEP_DIRECT_P2G", "STEP_GRAPH", "STEP_RESORT", "STEP_TIMINGS",
-    "STEP_SAMPLED", "STEP_UNFUSED", "Batch", "IcempmError", "Partition", "EXPORTED_SYMBOLS", "LIB_PATH", "lib", "params_derive",
+    "STEP_SAMPLED", "STEP_SPAN", "STEP_UNFUSED", "Batch", "IcempmError", "Partition", "EXPORTED_SYMBOLS", "LIB_PATH", "lib", "params_derive",
     "params_parse_json", "params_reset", "prepare_raw_points", "OK", "ERR_ARGUMENT", "ERR_CAPACITY", "ERR_DEVICE",
     "ERR_HIP", "ERR_NOMEM", "ERR_STATE",
 ]

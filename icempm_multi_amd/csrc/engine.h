@@ -129,8 +129,10 @@ private:
     int launch_substeps(int nsub, int start_buf, bool fused, bool direct, bool timed, int sample = 1);
     struct GraphExec;
     // n units of one kind from grid buffer b and clock slot cs; unit j is timed
-    // when timed and (k0 + j) % sample == 0
-    int emit_units(int kind, int n, int b, int cs, bool timed, int sample, int k0);
+    // when timed and (k0 + j) % sample == 0.  With span, the n units of a chain
+    // share one event pair around them, counted as n launches (back-to-back
+    // launches, with no event between them).
+    int emit_units(int kind, int n, int b, int cs, bool timed, int sample, int k0, bool span = false);
     int chunk_graph(int kind, int n, int b, int cs, int tmode, bool first_timed, GraphExec *&out);
     int run_units(int kind, int n, int &b, int &cs, bool use_graph, int tmode, int k0,
                   std::vector<GraphExec *> &replayed);
@@ -140,7 +142,7 @@ private:
     void mark_buffers(bool fused, bool direct, int b, int nsub);
     int record(int cls, bool timed, hipError_t (*)(void *), void *);
     void timed_begin(bool timed);
-    void timed_end(bool timed, int cls);
+    void timed_end(bool timed, int cls, int launches = 1);
     int collect_timings();
     void record(hipEvent_t e);
     void invalidate_graphs();
@@ -189,6 +191,7 @@ private:
     // timings
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
     std::vector<int> ev_cls_;
+    std::vector<int> ev_w_;     // launches each event pair spans
     size_t ev_used_ = 0;
     double cls_ms_[kNumCls] = {0};
     long long cls_cnt_[kNumCls] = {0};
@@ -205,6 +208,7 @@ private:
         hipGraphExec_t exec = nullptr;
         std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
         std::vector<int> cls;
+        std::vector<int> w;
     };
     typedef std::array<int, 7> GraphKey;
     static constexpr int kMaxChunk = 64;      // units per graph: at most 192 kernel nodes (3 per unfused unit)

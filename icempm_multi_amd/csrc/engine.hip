@@ -706,15 +706,17 @@ void Engine::timed_begin(bool timed)
         hipEventCreate(&b);
         ev_pool_.push_back({a, b});
         ev_cls_.push_back(0);
+        ev_w_.push_back(1);
     }
     record(ev_pool_[ev_used_].first);
 }
 
-void Engine::timed_end(bool timed, int cls)
+void Engine::timed_end(bool timed, int cls, int launches)
 {
     if (!timed) return;
     record(ev_pool_[ev_used_].second);
     ev_cls_[ev_used_] = cls;
+    ev_w_[ev_used_] = launches;
     ev_used_++;
 }
 
@@ -726,7 +728,7 @@ int Engine::collect_timings()
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev_pool_[i].first, ev_pool_[i].second));
         cls_ms_[ev_cls_[i]] += ms;
-        cls_cnt_[ev_cls_[i]]++;
+        cls_cnt_[ev_cls_[i]] += ev_w_[i];
     }
     ev_used_ = 0;
     return ICEMPM_OK;
@@ -999,13 +1001,15 @@ int Engine::get_bins(int scene, int32_t *keys, int32_t *perm, int n, int32_t *ce
 // Chain or Tail unit forms the node velocities of grid b from its momenta (the
 // grid update folded in), reading clock slot cs and writing the advanced
 // clock to cs^1, which the later launches of the substep read.
-int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sample, int k0)
+int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sample, int k0, bool span)
 {
     StepArgs a = args();
     SceneClock *slot[2] = {d_clocks_, d_clocks_ + nscenes()};
     const int nb_upd = update_blocks(n_chunks_);
+    const bool span_chain = span && timed_all && kind == kUnitChain && n > 0;
+    timed_begin(span_chain);
     for (int j = 0; j < n; j++) {
-        const bool timed = timed_all && (sample <= 1 || (k0 + j) % sample == 0);
+        const bool timed = !span_chain && timed_all && (sample <= 1 || (k0 + j) % sample == 0);
         StepArgs au = a;               // the substep's clock
         au.clocks = slot[cs];
         au.clocks_next = slot[cs ^ 1];
@@ -1065,6 +1069,7 @@ int Engine::emit_units(int kind, int n, int b, int cs, bool timed_all, int sampl
             continue;
         }
     }
+    timed_end(span_chain, kClsFused, n);
     return ICEMPM_OK;
 }
 
@@ -1102,15 +1107,17 @@ int Engine::chunk_graph(int kind, int n, int b, int cs, int tmode, bool first_ti
         capturing_ = true;
         // first_timed with sampling: unit 0 of the chunk is a sampled substep
         // of its call, and so is every unit a multiple of the sample after it
-        const int rc = emit_units(kind, n, b, cs, tmode != 0, sample, first_timed ? 0 : 1);
+        const int rc = emit_units(kind, n, b, cs, tmode != 0, sample, first_timed ? 0 : 1, tmode == 3);
         capturing_ = false;
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(stream_, &g);
         GraphExec gx;
         gx.ev.assign(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
         gx.cls.assign(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
+        gx.w.assign(ev_w_.begin(), ev_w_.begin() + ev_used_);
         ev_pool_.erase(ev_pool_.begin(), ev_pool_.begin() + ev_used_);
         ev_cls_.erase(ev_cls_.begin(), ev_cls_.begin() + ev_used_);
+        ev_w_.erase(ev_w_.begin(), ev_w_.begin() + ev_used_);
         ev_used_ = 0;
         auto drop = [&]() {
             for (auto &e : gx.ev) {
@@ -1137,7 +1144,7 @@ int Engine::read_graph_events(const GraphExec &gx)
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, gx.ev[i].first, gx.ev[i].second));
         cls_ms_[gx.cls[i]] += ms;
-        cls_cnt_[gx.cls[i]]++;
+        cls_cnt_[gx.cls[i]] += gx.w[i];
     }
     return ICEMPM_OK;
 }
@@ -1149,7 +1156,7 @@ int Engine::run_units(int kind, int n, int &b, int &cs, bool use_graph, int tmod
 {
     const int sample = tmode == 2 ? kTimingSample : 1;
     if (!use_graph) {
-        RC(emit_units(kind, n, b, cs, tmode != 0, sample, k0));
+        RC(emit_units(kind, n, b, cs, tmode != 0, sample, k0, tmode == 3));
         advance_units(kind, n, b, cs);
         return ICEMPM_OK;
     }
@@ -1250,7 +1257,9 @@ int Engine::step(int nsub, int flags)
     const bool fused = !(flags & ICEMPM_STEP_UNFUSED);
     const bool timed = (flags & ICEMPM_STEP_TIMINGS) != 0;
     const bool use_graph = (flags & ICEMPM_STEP_GRAPH) != 0;
-    const int tmode = timed ? ((flags & ICEMPM_STEP_SAMPLED) ? 2 : 1) : 0;
+    // timing: 1 an event pair per launch, 2 the same for one substep in
+    // kTimingSample, 3 one pair around each run of chain launches
+    const int tmode = timed ? ((flags & ICEMPM_STEP_SAMPLED) ? 2 : (flags & ICEMPM_STEP_SPAN) ? 3 : 1) : 0;
 
     // A call captures at most kMaxGraphsPerCall chunk graphs; the cache is
     // cleared between calls (the last call's replays have completed: it

@@ -89,6 +89,7 @@ STEP_TIMINGS = 0x4
 STEP_DIRECT_P2G = 0x8
 STEP_UNFUSED = 0x10
 STEP_SAMPLED = 0x20
+STEP_SPAN = 0x40
 
 # status codes of every icempm_* call (ICEMPM_OK, ICEMPM_ERR_* of include/icempm.h)
 OK = 0

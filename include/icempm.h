@@ -127,6 +127,11 @@ typedef struct icempm_simparams {
                                          substep in 32 only (substeps 0, 32, ... of each call; kernel
                                          stats then count those launches); with GRAPH|TIMINGS the
                                          events are recorded inside the graph */
+#define ICEMPM_STEP_SPAN        0x40  /* with TIMINGS: one event pair around each run of a call's
+                                         substep launches (a graph chunk's, or all of them with
+                                         direct launches), counted as that many launches; the
+                                         kernel stats then give the mean of back-to-back launches
+                                         with no event between them (other phases as TIMINGS) */
 
 /* Library identification and the last error text of the calling thread. */
 const char *icempm_version(void);
@@ -207,7 +212,8 @@ int icempm_partition_get_point_count(icempm_partition *p, int *n_points, int *n_
 int icempm_partition_get_timings(icempm_partition *p, float *out, int n);
 
 /* Device time per kernel class since the last reset_indenter_force_accumulator,
- * recorded with hipEvents around each launch when ICEMPM_STEP_TIMINGS is set:
+ * recorded with hipEvents around each launch when ICEMPM_STEP_TIMINGS is set
+ * (around each run of back-to-back fused launches with ICEMPM_STEP_SPAN):
  * class 0 clock, 1 zero-fill, 2 P2G, 3 fused G2P+P2G (the grid update folded
  * in), 4 G2P (the last of a binned call also folds the grid update in), 5 grid
  * update (unbinned and unfused stepping, and the phase calls), 6 sort.

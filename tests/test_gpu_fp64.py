@@ -259,6 +259,36 @@ def test_sampled_timings_count_launches(graph):
     assert part.error_code == 0
 
 
+@pytest.mark.parametrize("graph", [0, im.STEP_GRAPH])
+def test_span_timings_count_launches(graph):
+    """One event pair around each run of substep launches (a graph chunk's,
+    or the call's with direct launches), counted as that many launches: a
+    70-substep call is the head P2G, 69 chain launches and the last substep.
+    The mean of the back-to-back launches is of the size a per-launch event
+    pair measures."""
+    P, h, n = small_block(seed=0x1CE0202)
+    part = make_partition(P, h, n)
+    part.sort_points()
+    part.reset_indenter_force_accumulator()
+    part.step(70, graph | im.STEP_TIMINGS | im.STEP_SPAN)
+    st = part.kernel_stats()
+    assert st["p2g"][1] == 1 and st["fused_g2p_p2g"][1] == 69 and st["g2p"][1] == 1
+    assert st["update"][1] == 0
+    span_mean = st["fused_g2p_p2g"][0] / 69
+    part.step(70, graph | im.STEP_TIMINGS | im.STEP_SPAN)
+    st = part.kernel_stats()
+    assert st["p2g"][1] == 2 and st["fused_g2p_p2g"][1] == 138 and st["g2p"][1] == 2
+    part.reset_indenter_force_accumulator()
+    part.step(70, graph | im.STEP_TIMINGS)
+    st = part.kernel_stats()
+    assert st["fused_g2p_p2g"][1] == 69
+    each_mean = st["fused_g2p_p2g"][0] / 69
+    assert 0.2 * each_mean < span_mean < 2.0 * each_mean, (span_mean, each_mean)
+    part.step(70, graph)   # untimed: no more events
+    assert part.kernel_stats()["fused_g2p_p2g"][1] == 69
+    assert part.error_code == 0
+
+
 @pytest.mark.parametrize("dynamic", [0, 1])
 @pytest.mark.parametrize("lane_map", [0, 1, 2])
 @pytest.mark.parametrize("tile", [(4, 32), (8, 16), (16, 16)])
